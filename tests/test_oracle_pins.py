@@ -1,0 +1,391 @@
+"""Pins of the CPU oracle against known-answer vectors, exact arithmetic,
+brute force and closed forms (CPU only).
+
+The reference ships no golden vectors or known-answer tests for this path
+(its tests/ directory holds benchmarks only), so each oracle function is
+pinned here by something independent of the oracle's own code.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import numpy_oracle as NO
+from haissinskijulia_amd import workloads as W
+
+
+# --------------------------------------------------------------------------
+# Philox4x32-10 and the QE normal stream
+# --------------------------------------------------------------------------
+
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32_10
+    assert O.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert O.philox4x32_10([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert O.philox4x32_10([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+                           [0xa4093822, 0x299f31d0]) == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def _bm_from_words(x):
+    u1 = ((x[0] >> 5) * 67108864 + (x[1] >> 6) + 1) * 2.0 ** -53
+    u2 = ((x[2] >> 5) * 67108864 + (x[3] >> 6)) * 2.0 ** -53
+    r = math.sqrt(-2.0 * math.log(u1))
+    th = (2 * math.pi) * u2
+    return r * math.cos(th), r * math.sin(th)
+
+
+def test_normals_follow_box_muller_of_philox_words():
+    seed, turn = 0x1234_5678_9abc, 17
+    nrm = O.normals(seed, turn, 6, 10)  # particles 6..15
+    for i in range(6, 16):
+        j = i >> 1
+        x = O.philox4x32_10([j & 0xffffffff, j >> 32, turn, 0], [seed & 0xffffffff, seed >> 32])
+        a, b = _bm_from_words(x)
+        assert nrm[i - 6] == pytest.approx(b if i & 1 else a, rel=1e-15, abs=1e-15)
+
+
+def test_normals_are_standard_normal():
+    from scipy import stats
+    x = O.normals(42, 3, 0, 200_000)
+    assert abs(x.mean()) < 4 * 1 / math.sqrt(x.size)
+    assert abs(x.var() - 1) < 4 * math.sqrt(2 / x.size)
+    assert stats.kstest(x, "norm").pvalue > 1e-3
+    # pairs of one block are independent
+    assert abs(np.corrcoef(x[0::2], x[1::2])[0, 1]) < 4 / math.sqrt(x.size / 2)
+
+
+def test_generate_gaussian_uses_stream_one():
+    z, dE = O.generate_gaussian(9, 0.0, 4e-3, 0.0, 3e6, 5)
+    for i in range(5):
+        x = O.philox4x32_10([i, 0, 0, 1], [9, 0])
+        a, b = _bm_from_words(x)
+        assert z[i] == pytest.approx(4e-3 * a, rel=1e-15)
+        assert dE[i] == pytest.approx(3e6 * b, rel=1e-15)
+
+
+# --------------------------------------------------------------------------
+# Bin grid (evolution.jl:270-274) -- exact rational rounding
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("zw,nb", [(0.03, 1024), (7.5 * 4.123456789e-3, 1000), (1e-6, 7),
+                                   (0.0123, 1), (0.05, 16384)])
+def test_bin_grid_is_correctly_rounded(zw, nb):
+    edges, centers, step = O.bin_grid(zw, nb)
+    fz = Fraction(zw)
+    ks = list(range(0, nb + 1, max(1, nb // 97))) + [nb]
+    for k in ks:
+        assert edges[k] == float(fz * (2 * k - nb) / nb)
+    for i in list(range(0, nb, max(1, nb // 89))) + [nb - 1]:
+        assert centers[i] == float(fz * (2 * i + 1 - nb) / nb)
+    assert edges[0] == -zw and edges[-1] == zw
+    assert step == float(2 * fz / nb)
+    assert np.all(np.diff(edges) > 0)
+    np.testing.assert_array_equal(centers, -centers[::-1])
+
+
+# --------------------------------------------------------------------------
+# compute_global_std (utils.jl:569-606)
+# --------------------------------------------------------------------------
+
+def test_global_std_closed_form_and_edges():
+    assert O.global_std([1.0, 2.0, 3.0, 4.0]) == math.sqrt(5 / 3)
+    assert O.global_std([5.0]) == 0.0
+    assert O.global_std([]) == 0.0
+    assert O.global_std([2.0, 2.0, 2.0]) == 0.0
+    rng = np.random.default_rng(1)
+    x = rng.normal(0.3, 2.0, 10_001)
+    assert O.global_std(x) == pytest.approx(np.std(x, ddof=1), rel=1e-12)
+    assert O.serial_std(x) == pytest.approx(np.std(x, ddof=1), rel=1e-14)
+    assert math.isnan(O.global_std([1.0, float("nan")]))
+
+
+# --------------------------------------------------------------------------
+# Histogram (utils.jl:351-401)
+# --------------------------------------------------------------------------
+
+def test_histogram_brute_force_and_edge_cases():
+    nb = 37
+    edges, centers, step = O.bin_grid(0.02, nb)
+    inv = 1.0 / step
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-0.025, 0.025, 20_000)
+    x[:nb + 1] = edges                         # every edge exactly
+    x[nb + 1] = edges[-1]                      # last edge -> last bin
+    x[nb + 2:nb + 6] = [np.nan, np.inf, -np.inf, np.nextafter(edges[-1], 1)]
+    c = O.histogram(x, edges[0], edges[-1], inv, nb)
+    assert c.sum() == np.count_nonzero((x >= edges[0]) & (x <= edges[-1]))
+    np.testing.assert_array_equal(c, NO.histogram(x, edges[0], edges[-1], inv, nb))
+    # the floor formula equals the true edge interval away from the edges
+    ok = (x >= edges[0]) & (x <= edges[-1])
+    for v in x[ok][:4000]:
+        b = int(np.searchsorted(edges, v, side="right")) - 1
+        b = min(b, nb - 1)
+        if min(abs(v - edges[b]), abs(v - edges[min(b + 1, nb)])) > 1e-9 * step:
+            assert math.floor((v - edges[0]) * inv) == b
+    one = O.histogram(np.array([edges[-1]]), edges[0], edges[-1], inv, nb)
+    assert one[-1] == 1 and one.sum() == 1
+
+
+# --------------------------------------------------------------------------
+# FFT and the wake potential (wakefield.jl:186-319)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("L", [2, 8, 1024, 4096])
+def test_fft_matches_numpy_and_parseval(L):
+    rng = np.random.default_rng(L)
+    re, im = rng.normal(size=L), rng.normal(size=L)
+    fr, fi = O.fft(re, im)
+    ref = np.fft.fft(re + 1j * im)
+    assert np.linalg.norm(fr + 1j * fi - ref) <= 1e-14 * np.linalg.norm(ref) * math.log2(L + 1)
+    assert np.sum(fr ** 2 + fi ** 2) == pytest.approx(L * np.sum(re ** 2 + im ** 2), rel=1e-12)
+    br, bi = O.fft(fr, fi, inverse=True)
+    np.testing.assert_allclose(br, re, atol=1e-13)
+    np.testing.assert_allclose(bi, im, atol=1e-13)
+
+
+@pytest.mark.parametrize("nb", [1, 2, 5, 64, 1000, 1024])
+def test_wake_potential_equals_direct_convolution(nb):
+    zw = 0.03
+    edges, centers, step = O.bin_grid(zw, nb)
+    rng = np.random.default_rng(nb)
+    counts = rng.integers(0, 1000, nb).astype(np.int64)
+    b = 0.25e-3
+    wf = ((120 * math.pi) * W.SPEED_LIGHT) / (math.pi * b)
+    ws = math.sqrt((2 * 30.0) / b)
+    wt = np.array([O.wake_function(c, wf, ws, 1 / 4e-3) for c in centers])
+    lt = np.array([O.delta(c, 0.15) for c in centers])
+    cur = 3.7e-11
+    P = O.wake_potential(counts, wt, lt, cur)
+    Pd = O.wake_potential(counts, wt, lt, cur, direct=True)
+    g = lt * (counts * (1 / counts.sum()))
+    Pn = np.convolve(wt, g)[:nb] * cur
+    assert np.linalg.norm(P - Pd) <= 1e-13 * np.linalg.norm(Pd)
+    assert np.linalg.norm(Pn - Pd) <= 1e-13 * np.linalg.norm(Pd)
+    np.testing.assert_allclose(P, NO.potential_fft(counts, wt, lt, cur), rtol=0,
+                               atol=1e-13 * np.abs(Pd).max())
+    # W vanishes for positive centres (causality, wakefield.jl:117)
+    assert np.all(wt[centers > 0] == 0)
+
+
+def test_wake_potential_zero_counts_is_zero():
+    P = O.wake_potential(np.zeros(16, dtype=np.int64), np.ones(16), np.ones(16), 1.0)
+    assert np.all(P == 0)
+
+
+def test_wake_current_heuristic():
+    # evolution.jl:377-386: pf = (1e11 / 10^floor(log10 N)) * N
+    E0, R, sz, eta, sE = 3e9, 37.24, 4e-3, 1.1e-3, 3e6
+    for N, pf in [(1, 1e11), (100_000, 1e11), (12345, 1e11 / 1e4 * 12345), (99_999, 1e11 / 1e4 * 99_999)]:
+        expect = (((pf / E0) / ((2 * math.pi) * R)) * sz) / (eta * (sE * sE))
+        assert O.wake_current(N, E0, R, sz, eta, sE) == expect
+
+
+def test_resonator_wake_closed_forms():
+    Rs, fr, Q = 10e3, 5e9, 1.0
+    a = 2 * math.pi * fr / (2 * Q)
+    assert O.resonator_wake(0.0, Rs, fr, Q) == pytest.approx(a * Rs, rel=1e-15)
+    # integral of W over s equals R_s * c * (area of the impedance real-part delta at DC) -> 0
+    s = np.linspace(0, 2.0, 400_001)
+    w = np.array([O.resonator_wake(v, Rs, fr, Q) for v in s[:2000]])
+    assert w[1] < 2 * a * Rs and w[1] > 0
+    # high Q: slowly decaying cosine with frequency ~ f_r
+    s1 = W.SPEED_LIGHT / 5e9
+    assert O.resonator_wake(s1, 1.0, 5e9, 1e6) == pytest.approx(2 * (math.pi * 5e9 / 1e6), rel=1e-5)
+
+
+# --------------------------------------------------------------------------
+# Interpolation (wakefield.jl:22-105)
+# --------------------------------------------------------------------------
+
+def test_interpolation_brute_force_and_end_quirk():
+    nb = 50
+    _, centers, _ = O.bin_grid(0.01, nb)
+    rng = np.random.default_rng(2)
+    P = rng.normal(size=nb)
+    z = np.concatenate([rng.uniform(-0.012, 0.012, 5000), centers, [np.inf, -np.inf]])
+    out = O.interpolate(z, centers, P)
+    np.testing.assert_array_equal(out, NO.interpolate(z, centers, P))
+    inner = (z > centers[0]) & (z < centers[-1])
+    np.testing.assert_allclose(out[inner], np.interp(z[inner], centers, P), rtol=0, atol=1e-13)
+    # z beyond the last centre: u = (c_nb - c_1) * inv -- the reference clamps the
+    # index to nb-1 and the weight is u - floor(u)
+    inv = 1.0 / ((centers[-1] - centers[0]) / (nb - 1))
+    u = (centers[-1] - centers[0]) * inv
+    k = min(int(math.floor(u)), nb - 2)
+    expect = P[k] * (1.0 - (u - math.floor(u))) + P[k + 1] * (u - math.floor(u))
+    assert O.interpolate(np.array([1.0]), centers, P)[0] == expect
+    if u == nb - 1:   # the quirk: P[nb-2] (1-based P_{nb-1}), not P[nb-1]
+        assert expect == P[nb - 2]
+    assert O.interpolate(np.array([-1.0]), centers, P)[0] == P[0]
+    assert np.all(O.interpolate(z, centers[:1], P[:1]) == P[0])
+
+
+# --------------------------------------------------------------------------
+# Stage kernels against the independent numpy restatement
+# --------------------------------------------------------------------------
+
+def _lattice():
+    return W.ring_3gev_lattice()
+
+
+def test_stage_kernels_match_numpy_restatement():
+    p = _lattice()
+    rng = np.random.default_rng(11)
+    n = 3000
+    z = rng.normal(0, 4e-3, n)
+    dE = rng.normal(0, 3e6, n)
+    r = rng.normal(size=n)
+    E0 = p["E0"]
+    g0 = E0 / p["mass"]
+    b0 = math.sqrt(max(0.0, 1.0 - 1.0 / (g0 * g0)))
+    rf = O.calc_rf_factor(p["freq_rf"], b0)
+    assert rf == (p["freq_rf"] * (2 * math.pi)) / (b0 * W.SPEED_LIGHT)
+    sp = math.sin(p["phi_s"])
+    tol = dict(rtol=4e-16, atol=0)
+    np.testing.assert_allclose(O.rf_kick(p["voltage"], sp, rf, p["phi_s"], z, dE),
+                               NO.rf_kick(p["voltage"], sp, rf, p["phi_s"], z, dE), rtol=1e-15, atol=1e-7)
+    np.testing.assert_array_equal(O.quantum_excitation(E0, p["radius"], 3e6, r, dE),
+                                  NO.quantum_excitation(E0, p["radius"], 3e6, r, dE))
+    np.testing.assert_array_equal(O.synchrotron_radiation(E0, p["radius"], dE),
+                                  NO.synchrotron_radiation(E0, p["radius"], dE))
+    np.testing.assert_allclose(
+        O.drift_update_eta(z, dE, p["mass"], g0, p["alpha_c"], p["harmonic"], b0, E0, rf, p["phi_s"]),
+        NO.drift_update_eta(z, dE, p["mass"], g0, p["alpha_c"], p["harmonic"], b0, E0, rf, p["phi_s"]), **tol)
+    zs, skipped = O.apply_phase_advance(z, dE, 1e-3, p["harmonic"], 0.0, E0, rf, p["phi_s"])
+    assert skipped and np.array_equal(zs, z)
+
+
+def test_sr_decrement_matches_damping_time():
+    # U0-free SR of synchrotron_radiation.jl:44: 1/(4*8.85e-5*27/R) ~ 3896 turns
+    p = _lattice()
+    d = 4 * 8.85e-5 * (p["E0"] / 1e9) ** 3 / p["radius"]
+    assert 1 / d == pytest.approx(3896, rel=2e-3)
+
+
+# --------------------------------------------------------------------------
+# Whole loop: closed forms of the linear map, statistics, rank independence
+# --------------------------------------------------------------------------
+
+def _one_turn_matrix(p, with_sr):
+    E0 = p["E0"]
+    g0 = E0 / p["mass"]
+    b0 = math.sqrt(1 - 1 / (g0 * g0))
+    eta0 = p["alpha_c"] - 1 / (g0 * g0)
+    rf = (p["freq_rf"] * (2 * math.pi)) / (b0 * W.SPEED_LIGHT)
+    coeff = ((2 * math.pi * p["harmonic"]) * eta0) / ((b0 * b0) * E0)
+    K = coeff / rf
+    a = p["voltage"] * math.cos(p["phi_s"]) * rf
+    d = 1 - 4 * 8.85e-5 * (E0 / 1e9) ** 3 / p["radius"] if with_sr else 1.0
+    A = np.array([[1 + K * d * a, -K * d], [-d * a, d]])
+    return A, K, a, d
+
+
+def test_synchrotron_tune_of_linear_map():
+    p = _lattice()
+    p.update(n_turns=1, use_wakefield=False, update_eta=False, update_E0=False,
+             SR_damping=False, use_excitation=False)
+    z = np.array([1e-7, -2e-7])
+    dE = np.array([0.0, 50.0])
+    zs = [z.copy()]
+    for _ in range(3):
+        z, dE, res = O.evolve(z, dE, 0, p)
+        assert res.status == 0
+        zs.append(z.copy())
+    A, K, a, _ = _one_turn_matrix(p, False)
+    tr = (zs[2][0] + zs[0][0]) / zs[1][0]
+    assert tr == pytest.approx(2 + K * a, rel=1e-7)
+    Qs = math.acos(1 + K * a / 2) / (2 * math.pi)
+    assert Qs == pytest.approx(0.0081, rel=0.05)
+
+
+def test_sr_damping_shrinks_phase_space_area_by_d_squared():
+    p = _lattice()
+    T = 200
+    p.update(n_turns=T, use_wakefield=False, update_eta=False, update_E0=False,
+             SR_damping=True, use_excitation=False)
+    rng = np.random.default_rng(3)
+    z0 = rng.normal(0, 1e-6, 500)
+    e0 = rng.normal(0, 600.0, 500)
+    z1, e1, res = O.evolve(z0, e0, 0, p)
+    A, _, _, d = _one_turn_matrix(p, True)
+    det0 = np.linalg.det(np.cov(np.vstack([z0, e0])))
+    det1 = np.linalg.det(np.cov(np.vstack([z1, e1])))
+    assert det1 / det0 == pytest.approx(d ** (2 * T), rel=1e-6)
+
+
+def test_qe_sr_covariance_follows_linear_lyapunov_recursion():
+    p = _lattice()
+    T = 300
+    p.update(n_turns=T, use_wakefield=False, update_eta=False, update_E0=False,
+             SR_damping=True, use_excitation=True)
+    n = 40_000
+    rng = np.random.default_rng(4)
+    z0 = rng.normal(0, 5e-7, n)
+    e0 = rng.normal(0, 3e5 * 0.1, n)
+    z1, e1, res = O.evolve(z0, e0, 0, p, seed=77)
+    A, K, a, d = _one_turn_matrix(p, True)
+    sE0 = O.global_std(e0)
+    x = p["E0"] / 1e9
+    dd = ((4 * 8.85e-5) * (x * x * x)) / p["radius"]
+    exc = math.sqrt(1 - (1 - dd) ** 2) * sE0
+    b = np.array([-K * d * exc, d * exc])
+    S = np.cov(np.vstack([z0, e0]))
+    for _ in range(T):
+        S = A @ S @ A.T + np.outer(b, b)
+    Sm = np.cov(np.vstack([z1, e1]))
+    for i, j in [(0, 0), (1, 1), (0, 1)]:
+        se = math.sqrt((S[i, i] * S[j, j] + S[i, j] ** 2) / n)
+        assert abs(Sm[i, j] - S[i, j]) < 5 * se, (i, j, Sm[i, j], S[i, j], se)
+
+
+def test_evolve_rank_count_independent():
+    p = W.ring_3gev_lattice(n_turns=20)
+    z0, e0 = O.generate_gaussian(5, 0.0, 4e-3, 0.0, 3e6, 5000)
+    a = O.evolve(z0, e0, 256, p, seed=3, nranks=1)
+    b = O.evolve(z0, e0, 256, p, seed=3, nranks=3)
+    for u, v in [(a[0], b[0]), (a[1], b[1])]:
+        assert np.max(np.abs(u - v)) <= 1e-12 * np.sqrt(np.mean(u * u))
+    assert a[2].E0 == pytest.approx(b[2].E0, rel=1e-14)
+
+
+@pytest.mark.parametrize("flags", [(True, True, True, True, True), (True, False, True, True, True),
+                                   (True, True, False, False, False), (False, True, True, True, True)])
+@pytest.mark.parametrize("turns", [1, 5])
+def test_evolve_matches_numpy_restatement(flags, turns):
+    p = W.ring_3gev_lattice(n_turns=turns, flags=flags)
+    n, nb = 256, 32
+    z0, e0 = O.generate_gaussian(8, 0.0, 4e-3, 0.0, 3e6, n)
+    nrm = np.vstack([O.normals(8, t, 0, n) for t in range(turns)])
+    z1, e1, res = O.evolve(z0, e0, nb, p, normals=nrm)
+    assert res.status == 0
+    z2, e2, sE, sz, E0 = NO.evolve(z0, e0, nb, p, nrm)
+    assert np.max(np.abs(z1 - z2)) <= 1e-13 * np.sqrt(np.mean(z2 ** 2))
+    assert np.max(np.abs(e1 - e2)) <= 1e-13 * np.sqrt(np.mean(e2 ** 2))
+    assert res.E0 == pytest.approx(E0, rel=1e-15)
+    assert res.sigma_E == pytest.approx(sE, rel=1e-12)
+    assert res.sigma_z == pytest.approx(sz, rel=1e-12)
+
+
+def test_evolve_philox_path_equals_injected_stream():
+    p = W.ring_3gev_lattice(n_turns=3)
+    n = 101
+    z0, e0 = O.generate_gaussian(1, 0.0, 4e-3, 0.0, 3e6, n)
+    nrm = np.vstack([O.normals(21, 10 + t, 0, n) for t in range(3)])
+    a = O.evolve(z0, e0, 64, p, normals=nrm)
+    b = O.evolve(z0, e0, 64, p, seed=21, turn_offset=10)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_evolve_error_cases():
+    p = W.ring_3gev_lattice(n_turns=1)
+    z0, e0 = O.generate_gaussian(1, 0.0, 4e-3, 0.0, 3e6, 10)
+    assert O.evolve(z0, e0, 0, p)[2].status == -3          # nbins <= 0 with the wake
+    q = dict(p, pipe_radius=0.0)
+    assert O.evolve(z0, e0, 16, q)[2].status == -4         # pipe_radius <= 0
+    assert O.evolve(z0[:0], e0[:0], 16, p)[2].status == -2  # log10(0) with the wake
+    q = dict(p, use_wakefield=False)
+    r = O.evolve(z0[:0], e0[:0], 16, q)[2]
+    assert r.status == 0 and math.isnan(r.E0)             # 0/0 mean in the E0 update
