@@ -1,0 +1,824 @@
+// hm_engine.hip -- host side of the MI355X tracker: contexts, device-resident
+// bunches, the batched turn loop and the evolve entry points of the C ABI.
+//
+// One evolve call (src/evolution.jl:206-471) runs as:
+//   set-up   k_moments + k_moments_final   sigma_E0, sigma_z0 (evolution.jl:245-253)
+//            host: bin grid, wake and line-density tables (evolution.jl:261-281,
+//            wakefield.jl:194-221), FFT block plan; k_fft_wake; k_init
+//   turns    k_turn_pass, k_conv, [k_interp_sum], k_finalize   per turn
+//   epilogue k_moments(mode 1) + k_moments_final            (evolution.jl:461-470)
+// A batch of bunches shares every launch: tiles, convolution items and
+// finalize blocks are looked up through per-launch tables.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <memory>
+
+#include "hm_internal.hpp"
+
+namespace hm {
+
+static thread_local std::string g_last_error;
+
+hm_status fail(hm_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+hm_status set_device(hm_ctx *ctx) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    return HM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers: exact bin grid, tables, twiddles
+// ---------------------------------------------------------------------------
+
+static int bit_length_u128(unsigned __int128 x) {
+    int b = 0;
+    while (x) { x >>= 1; ++b; }
+    return b;
+}
+
+// RN(x * num / den), exact: x = M 2^E with a 53-bit integer M; the quotient
+// is formed in 128-bit integers with >= 55 bits and rounded half-to-even
+// with a sticky bit.
+double rn_ratio(double x, int64_t num, int64_t den) {
+    if (x == 0.0 || num == 0) return 0.0;
+    const bool neg = (x < 0) != (num < 0);
+    int e;
+    const double f = std::frexp(std::fabs(x), &e);  // |x| = f 2^e, f in [0.5, 1)
+    const uint64_t M = (uint64_t)std::ldexp(f, 53);
+    const int E = e - 53;
+    const unsigned __int128 N = (unsigned __int128)M * (unsigned __int128)(num < 0 ? -num : num);
+    const unsigned __int128 D = (unsigned __int128)den;
+    int s = 56 + bit_length_u128(D) - bit_length_u128(N);
+    if (s < 0) s = 0;
+    const unsigned __int128 Ns = N << s;
+    const unsigned __int128 Q = Ns / D;
+    const bool sticky = (Ns % D) != 0;
+    int r = bit_length_u128(Q) - 53;
+    unsigned __int128 mant = Q >> r;
+    const unsigned __int128 rem = Q & ((((unsigned __int128)1) << r) - 1);
+    const unsigned __int128 half = ((unsigned __int128)1) << (r - 1);
+    if (rem > half || (rem == half && (sticky || (mant & 1)))) {
+        mant += 1;
+        if (mant >> 53) { mant >>= 1; ++r; }
+    }
+    const double v = std::ldexp((double)(uint64_t)mant, E - s + r);
+    return neg ? -v : v;
+}
+
+// range(-zw, zw, length=nb+1) and its centres (evolution.jl:273-274),
+// correctly rounded; step(bin_edges) = RN(2 zw / nb)
+void bin_grid(double zw, int64_t nb, double *edges, double *centers, double *step) {
+    if (edges) {
+        for (int64_t k = 0; k <= nb; ++k) edges[k] = rn_ratio(zw, 2 * k - nb, nb);
+        edges[0] = -zw;
+        edges[nb] = zw;
+    }
+    if (centers)
+        for (int64_t i = 0; i < nb; ++i) centers[i] = rn_ratio(zw, 2 * i + 1 - nb, nb);
+    if (step) *step = (2.0 * zw) / (double)nb;
+}
+
+static double jl_max_h(double a, double b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    return (b > a) ? b : a;
+}
+
+// wakefield.jl:115-118
+static double ref_wake(double z, double wf, double ws, double inv_ctau) {
+    return z > 0 ? 0.0 : (wf * std::exp(z * inv_ctau)) * std::cos(ws * z);
+}
+
+// utils.jl:58-66
+static double ref_delta(double x, double sigma) {
+    const double s_inv = 1.0 / (std::sqrt(kTwoPi) * sigma);
+    const double ef = -0.5 / (sigma * sigma);
+    return s_inv * std::exp((x * x) * ef);
+}
+
+// broadband resonator wake [V/C] at distance s >= 0 behind the source (Chao,
+// Physics of Collective Beam Instabilities, eq. 2.84); W(0) is half the jump
+static double resonator_wake(double s, double Rs, double fr, double Q) {
+    const double wr = kTwoPi * fr;
+    const double a = wr / (2.0 * Q);
+    if (s < 0.0) return 0.0;
+    if (s == 0.0) return a * Rs;
+    const double tau = s / kSpeedLight;
+    const double env = 2.0 * a * Rs * std::exp(-a * tau);
+    const double d = wr * wr - a * a;
+    if (d > 0.0) {
+        const double wb = std::sqrt(d);
+        return env * (std::cos(wb * tau) - (a / wb) * std::sin(wb * tau));
+    } else if (d < 0.0) {
+        const double wb = std::sqrt(-d);
+        return env * (std::cosh(wb * tau) - (a / wb) * std::sinh(wb * tau));
+    }
+    return env * (1.0 - a * tau);
+}
+
+// evolution.jl:377-383
+double particle_factor(int64_t N) {
+    static const double p10[11] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10};
+    const double lg = std::log10((double)N);
+    const int64_t fl = lg >= 0 ? (int64_t)std::floor(lg) : (int64_t)std::ceil(lg - 1.0);
+    const int64_t pw = fl < -10 ? -10 : (fl > 10 ? 10 : fl);
+    const double den = pw >= 0 ? p10[pw] : 1.0 / p10[-pw];
+    return (1e11 / den) * (double)N;
+}
+
+int ilog2(int64_t x) {
+    int l = 0;
+    while ((int64_t(1) << l) < x) ++l;
+    return l;
+}
+
+// Largest convolution block whose 2B-point complex FFT fits the LDS of one
+// workgroup (kMaxFFT points at most).
+int max_fft_block(const hm_ctx *ctx) {
+    int Bk = kMaxFFT / 2;
+    while (Bk > 64 && (size_t)(2 * Bk) * 16 + 1024 > (size_t)ctx->lds_max) Bk >>= 1;
+    return Bk;
+}
+
+// Convolution block size: the wake support fits one block when it can, so
+// every output block needs one forward and one inverse FFT.
+int auto_fft_block(int64_t nw, int64_t nbins, int max_block) {
+    (void)nbins;
+    int Bk = 64;
+    while (Bk < nw && Bk < max_block) Bk <<= 1;
+    return Bk;
+}
+
+hm_status twiddles(hm_ctx *ctx, int F, const double **out) {
+    auto &buf = ctx->twiddles[F];
+    if (!buf.p) {
+        HM_TRY(buf.ensure((size_t)F));  // F/2 complex
+        std::vector<double> h((size_t)F);
+        const long double tp = 6.283185307179586476925286766559L;
+        for (int k = 0; k < F / 2; ++k) {
+            const long double a = tp * (long double)k / (long double)F;
+            h[2 * k] = (double)cosl(a);
+            h[2 * k + 1] = -(double)sinl(a);
+        }
+        HIP_TRY(hipMemcpy(buf.p, h.data(), sizeof(double) * (size_t)F, hipMemcpyHostToDevice));
+    }
+    *out = buf.p;
+    return HM_OK;
+}
+
+hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t nbins) {
+    if (!p) return fail(HM_ERR_ARG, "params is NULL");
+    if (n < 0) return fail(HM_ERR_ARG, "negative particle count");
+    if (p->n_turns < 0) return fail(HM_ERR_ARG, "n_turns must be >= 0");
+    if (n >= (int64_t(1) << 32)) return fail(HM_ERR_LIMIT, "at most 2^32-1 particles per bunch");
+    if (p->use_wakefield) {
+        if (nbins <= 0) return fail(HM_ERR_NBINS, "Buffer nbins invalid (must be > 0).");
+        if (nbins > (int64_t(1) << 22)) return fail(HM_ERR_LIMIT, "at most 2^22 bins");
+        const bool ext = o && (o->wake_table || o->n_resonators > 0);
+        if (!ext && p->pipe_radius <= 0)
+            return fail(HM_ERR_PIPE, "Wake parameter pipe_radius must be positive.");
+        if (n == 0)
+            return fail(HM_ERR_EMPTY_WAKE, "InexactError: log10 of zero particles (wake current)");
+        if (o && o->fft_block) {
+            const int fb = o->fft_block;
+            if (fb < 1 || (fb & (fb - 1)) || fb > kMaxFFT / 2)
+                return fail(HM_ERR_ARG, "fft_block must be a power of two <= %d", kMaxFFT / 2);
+        }
+    }
+    if (o && o->n_resonators > 0 && !o->resonators) return fail(HM_ERR_ARG, "resonators is NULL");
+    if (o && o->block_size && (o->block_size % 64 || o->block_size > 1024 || o->block_size < 64))
+        return fail(HM_ERR_ARG, "block_size must be a multiple of 64 in [64, 1024]");
+    return HM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+
+static int64_t pairs_per_tile_for(int64_t npairs, int threads) {
+    // tiling depends on the bunch size only, so a bunch is reduced in the same
+    // order alone and in any batch
+    const int64_t target_tiles = 1024;
+    int64_t ppt = (npairs + target_tiles - 1) / target_tiles;
+    ppt = ((ppt + threads - 1) / threads) * threads;
+    return std::max<int64_t>(ppt, threads);
+}
+
+Batch::~Batch() {
+    for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    d_bd_.release(); d_tiles_.release(); d_conv_.release(); d_list_.release();
+    d_tmp_tiles_.release(); d_tmp_list_.release();
+}
+
+int Batch::active_count() const {
+    int c = 0;
+    for (size_t i = 0; i < rb_.size(); ++i)
+        if (used_[i] && rb_[i].active) ++c;
+    return c;
+}
+
+hm_status Batch::upload_descriptors() {
+    HM_TRY(d_bd_.ensure(std::max<size_t>(rb_.size(), 1)));
+    std::vector<BunchDev> h(rb_.size());
+    for (size_t i = 0; i < rb_.size(); ++i) h[i] = rb_[i].d;
+    if (!h.empty())
+        HIP_TRY(hipMemcpyAsync(d_bd_.p, h.data(), h.size() * sizeof(BunchDev), hipMemcpyHostToDevice,
+                               ctx_->stream));
+    HIP_TRY(hipStreamSynchronize(ctx_->stream));  // h goes out of scope
+    return HM_OK;
+}
+
+// launch tables of the active bunches
+hm_status Batch::build_tables() {
+    std::vector<int32_t> tiles, conv, list;
+    pass_lds_ = 0;
+    Fmax_ = 2;
+    any_wake_ = any_wake_e0_ = false;
+    for (size_t i = 0; i < rb_.size(); ++i) {
+        RunBunch &q = rb_[i];
+        if (!used_[i] || !q.active) continue;
+        q.d.tile_begin = (int32_t)tiles.size();
+        for (int t = 0; t < q.d.ntiles; ++t) tiles.push_back((int32_t)i);
+        list.push_back((int32_t)i);
+        if (q.d.flags & F_WAKE) {
+            any_wake_ = true;
+            if (q.d.flags & F_E0) any_wake_e0_ = true;
+            for (int o = 0; o < q.d.nA; ++o) {
+                conv.push_back((int32_t)i);
+                conv.push_back(o);
+            }
+            Fmax_ = std::max(Fmax_, q.d.F);
+            if (!(q.d.flags & F_GHIST)) pass_lds_ = std::max(pass_lds_, (size_t)q.d.nbins * 4);
+        }
+    }
+    ntiles_ = (int)tiles.size();
+    nconv_ = (int)conv.size() / 2;
+    nlist_ = (int)list.size();
+    hipStream_t s = ctx_->stream;
+    HM_TRY(d_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
+    HM_TRY(d_conv_.ensure(std::max<size_t>(conv.size(), 2)));
+    HM_TRY(d_list_.ensure(std::max<size_t>(list.size(), 1)));
+    if (!tiles.empty())
+        HIP_TRY(hipMemcpyAsync(d_tiles_.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, s));
+    if (!conv.empty())
+        HIP_TRY(hipMemcpyAsync(d_conv_.p, conv.data(), conv.size() * 4, hipMemcpyHostToDevice, s));
+    if (!list.empty())
+        HIP_TRY(hipMemcpyAsync(d_list_.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+    HM_TRY(upload_descriptors());  // synchronises: the vectors above are still alive
+    dirty_ = false;
+    return HM_OK;
+}
+
+// k_moments over the tiles of `which` only (tile_begin relative to that table)
+hm_status Batch::run_subset_moments(const std::vector<int32_t> &which, int mode) {
+    if (which.empty()) return HM_OK;
+    hipStream_t s = ctx_->stream;
+    std::vector<int32_t> tiles;
+    std::vector<int32_t> saved;
+    for (int32_t i : which) {
+        saved.push_back(rb_[(size_t)i].d.tile_begin);
+        rb_[(size_t)i].d.tile_begin = (int32_t)tiles.size();
+        for (int t = 0; t < rb_[(size_t)i].d.ntiles; ++t) tiles.push_back(i);
+    }
+    HM_TRY(d_tmp_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
+    HM_TRY(d_tmp_list_.ensure(which.size()));
+    if (!tiles.empty())
+        HIP_TRY(hipMemcpyAsync(d_tmp_tiles_.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_tmp_list_.p, which.data(), which.size() * 4, hipMemcpyHostToDevice, s));
+    HM_TRY(upload_descriptors());
+    HIP_TRY(launch_moments(d_bd_.p, d_tmp_tiles_.p, (int)tiles.size(), mode, s));
+    HIP_TRY(launch_moments_final(d_bd_.p, d_tmp_list_.p, (int)which.size(), mode, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (size_t k = 0; k < which.size(); ++k) rb_[(size_t)which[k]].d.tile_begin = saved[k];
+    dirty_ = true;  // descriptors on the device carry the temporary tile_begin
+    return HM_OK;
+}
+
+hm_status Batch::epilogue(const std::vector<int32_t> &which) {
+    if (which.empty()) return HM_OK;
+    HM_TRY(run_subset_moments(which, 1));
+    for (int32_t i : which) {
+        rb_[(size_t)i].active = false;
+        rb_[(size_t)i].finished = true;
+    }
+    dirty_ = true;
+    return HM_OK;
+}
+
+hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) {
+    hipStream_t s = ctx_->stream;
+    std::vector<int32_t> fresh;
+    for (const RunBunch &spec : specs) {
+        HM_TRY(validate(&spec.p, &spec.o, spec.b->n, spec.nbins));
+        int slot = -1;
+        for (size_t i = 0; i < rb_.size(); ++i)
+            if (!used_[i]) { slot = (int)i; break; }
+        if (slot < 0) {
+            slot = (int)rb_.size();
+            rb_.emplace_back();
+            used_.push_back(false);
+        }
+        RunBunch &q = rb_[(size_t)slot];
+        q = spec;
+        used_[(size_t)slot] = true;
+        q.turns_done = 0;
+        q.n_turns = q.p.n_turns;
+        q.finished = false;
+        q.active = true;
+        hm_bunch *b = q.b;
+        BunchDev &d = q.d;
+        std::memset(&d, 0, sizeof(d));
+        const hm_params *p = &q.p;
+        d.z = b->z;
+        d.dE = b->dE;
+        d.n = b->n;
+        d.npairs = (b->n + 1) / 2;
+        d.pairs_per_tile = pairs_per_tile_for(d.npairs, threads_);
+        d.ntiles = (int32_t)((d.npairs + d.pairs_per_tile - 1) / d.pairs_per_tile);
+        uint32_t fl = 0;
+        if (p->use_wakefield) fl |= F_WAKE;
+        if (p->update_eta) fl |= F_ETA;
+        if (p->update_E0) fl |= F_E0;
+        if (p->SR_damping) fl |= F_SR;
+        if (p->use_excitation) fl |= F_QE;
+        if (q.o.normals && p->use_excitation) fl |= F_INJ;
+        if (q.o.charge > 0) fl |= F_CHARGE;
+        d.flags = fl;
+        d.mass = p->mass;
+        d.voltage = p->voltage;
+        d.alpha_c = p->alpha_c;
+        d.phis = p->phi_s;
+        d.sin_phis = std::sin(p->phi_s);
+        d.radius = p->radius;
+        d.freq_rf = p->freq_rf;
+        d.two_pi_h = kTwoPi * (double)p->harmonic;
+        d.pf = b->n > 0 ? particle_factor(b->n) : 0.0;
+        d.charge = q.o.charge;
+        d.E0_init = p->E0;
+        d.seed = q.o.seed;
+        d.turn_offset = q.o.turn_offset;
+        d.nbins = (int32_t)(p->use_wakefield ? q.nbins : 0);
+        Work &w = b->w;
+        HM_TRY(w.part.ensure((size_t)std::max(d.ntiles, 1) * 4));
+        HM_TRY(w.ipart.ensure((size_t)std::max(d.ntiles, 1)));
+        HM_TRY(w.moments.ensure((size_t)std::max(d.ntiles, 1) * 4));
+        HM_TRY(w.sc.ensure(1));
+        HM_TRY(w.out.ensure(16));
+        HIP_TRY(hipMemsetAsync(w.out.p, 0, 16 * sizeof(double), s));
+        d.part = w.part.p;
+        d.ipart = w.ipart.p;
+        d.moments = w.moments.p;
+        d.sc = w.sc.p;
+        d.out = w.out.p;
+        if (fl & F_INJ) {
+            const size_t cnt = (size_t)q.n_turns * (size_t)b->n;
+            HM_TRY(w.normals.ensure(cnt));
+            HIP_TRY(hipMemcpyAsync(w.normals.p, q.o.normals, cnt * sizeof(double),
+                                   hipMemcpyHostToDevice, s));
+            d.normals = w.normals.p;
+        }
+        q.o.normals = nullptr;  // host memory of the caller: not kept
+        q.o.wake_table = nullptr;
+        q.o.resonators = nullptr;
+        fresh.push_back((int32_t)slot);
+        if (idx) idx->push_back(slot);
+        // keep the wake description: the bin grid needs sigma_z0 first
+        q.wake_copy.clear();
+        q.res_copy.clear();
+        if (p->use_wakefield) {
+            if (spec.o.wake_table)
+                q.wake_copy.assign(spec.o.wake_table, spec.o.wake_table + spec.nbins);
+            else if (spec.o.n_resonators > 0)
+                q.res_copy.assign(spec.o.resonators, spec.o.resonators + 3 * spec.o.n_resonators);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s));  // normals copies read the caller's memory
+    // initial spreads (evolution.jl:245-253)
+    HM_TRY(run_subset_moments(fresh, 0));
+    std::vector<std::vector<double>> keep;   // host tables alive until the copies finish
+    for (int32_t i : fresh) {
+        RunBunch &q = rb_[(size_t)i];
+        BunchDev &d = q.d;
+        double o2[2];
+        HIP_TRY(hipMemcpy(o2, d.out, sizeof(o2), hipMemcpyDeviceToHost));
+        d.sigma_E0 = o2[0];
+        const double sz0 = o2[1];
+        if (!(d.flags & F_WAKE)) continue;
+        // evolution.jl:263-281
+        const int64_t nb = q.nbins;
+        const double hw = q.o.bin_half_width_sigmas > 0 ? q.o.bin_half_width_sigmas : 7.5;
+        double zw = hw * sz0;
+        if (zw < 1e-9) zw = 1e-6;
+        std::vector<double> centers((size_t)nb);
+        double step;
+        bin_grid(zw, nb, nullptr, centers.data(), &step);
+        d.e_first = -zw;
+        d.e_last = zw;
+        d.inv_step = 1.0 / step;                                              // utils.jl:378
+        d.c_first = centers[0];
+        d.c_last = centers[(size_t)nb - 1];
+        d.inv_s = nb > 1 ? 1.0 / ((d.c_last - d.c_first) / (double)(nb - 1)) : 0.0;  // wakefield.jl:39-42
+        std::vector<double> W((size_t)nb, 0.0), lam((size_t)nb);
+        const double wf = ((120.0 * kPi) * kSpeedLight) / (kPi * q.p.pipe_radius);   // evolution.jl:279
+        const double ws = std::sqrt(jl_max_h(0.0, (2.0 * 30.0) / q.p.pipe_radius));  // evolution.jl:280
+        const double inv_ctau = 1.0 / 4e-3;                                          // wakefield.jl:158
+        const double dsig = q.o.delta_sigma == 0.0 ? 0.15 : q.o.delta_sigma;
+        for (int64_t k = 0; k < nb; ++k) {
+            double wv;
+            if (!q.wake_copy.empty()) {
+                wv = q.wake_copy[(size_t)k];
+            } else if (!q.res_copy.empty()) {
+                const double sk = (double)k * step;
+                wv = 0.0;
+                for (size_t r = 0; r + 2 < q.res_copy.size(); r += 3)
+                    wv += resonator_wake(sk, q.res_copy[r], q.res_copy[r + 1], q.res_copy[r + 2]);
+            } else {
+                wv = ref_wake(centers[(size_t)k], wf, ws, inv_ctau);
+            }
+            W[(size_t)k] = wv;
+            lam[(size_t)k] = dsig > 0 ? ref_delta(centers[(size_t)k], dsig) : 1.0;
+        }
+        q.wake_copy.clear();
+        q.res_copy.clear();
+        int64_t nw = 0;
+        for (int64_t k = nb - 1; k >= 0; --k)
+            if (W[(size_t)k] != 0.0) { nw = k + 1; break; }
+        const int maxB = max_fft_block(ctx_);
+        if (q.o.fft_block > maxB)
+            return fail(HM_ERR_LIMIT, "fft_block %d exceeds the LDS limit (%d)", q.o.fft_block, maxB);
+        const int Bk = q.o.fft_block ? q.o.fft_block : auto_fft_block(nw, nb, maxB);
+        d.B = Bk;
+        d.F = 2 * Bk;
+        d.logF = ilog2(d.F);
+        d.nA = (int32_t)((nb + Bk - 1) / Bk);
+        d.nJ = (int32_t)((nw + Bk - 1) / Bk);
+        if ((size_t)nb * 4 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
+        Work &w = q.b->w;
+        HM_TRY(w.hist.ensure((size_t)nb));
+        HM_TRY(w.P.ensure((size_t)nb));
+        HM_TRY(w.lam.ensure((size_t)nb));
+        HM_TRY(w.Wtmp.ensure((size_t)nb));
+        HM_TRY(w.What.ensure((size_t)std::max(d.nJ, 1) * (size_t)d.F * 2));
+        HIP_TRY(hipMemcpyAsync(w.lam.p, lam.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(w.Wtmp.p, W.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemsetAsync(w.P.p, 0, (size_t)nb * 8, s));
+        keep.push_back(std::move(W));
+        keep.push_back(std::move(lam));
+        d.hist = w.hist.p;
+        d.P = w.P.p;
+        d.lam = w.lam.p;
+        d.What = w.What.p;
+        const double *tw;
+        HM_TRY(twiddles(ctx_, d.F, &tw));
+        d.tw = tw;
+    }
+    HM_TRY(upload_descriptors());   // also waits for the table copies above
+    keep.clear();
+    for (int32_t i : fresh) {
+        RunBunch &q = rb_[(size_t)i];
+        if (q.d.flags & F_WAKE)
+            HIP_TRY(launch_fft_wake(d_bd_.p + i, q.b->w.Wtmp.p, q.d.nJ, q.d.F, s));
+    }
+    HM_TRY(d_tmp_list_.ensure(fresh.size()));
+    HIP_TRY(hipMemcpyAsync(d_tmp_list_.p, fresh.data(), fresh.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_init(d_bd_.p, d_tmp_list_.p, (int)fresh.size(), s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int32_t> zero_turn;
+    for (int32_t i : fresh)
+        if (rb_[(size_t)i].n_turns == 0) zero_turn.push_back(i);
+    HM_TRY(epilogue(zero_turn));
+    dirty_ = true;
+    return HM_OK;
+}
+
+hm_status Batch::one_turn() {
+    hipStream_t s = ctx_->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (timing_) {
+        if (ev_used_ + 3 > ev_pool_.size()) {
+            if (ev_pool_.size() >= 3 * 4096) HM_TRY(flush_timing());
+            while (ev_used_ + 3 > ev_pool_.size()) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreate(&e));
+                ev_pool_.push_back(e);
+            }
+        }
+        e0 = ev_pool_[ev_used_];
+        e1 = ev_pool_[ev_used_ + 1];
+        e2 = ev_pool_[ev_used_ + 2];
+        ev_used_ += 3;
+        HIP_TRY(hipEventRecord(e0, s));
+    }
+    HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, ntiles_, pass_lds_, threads_, s));
+    if (timing_) HIP_TRY(hipEventRecord(e1, s));
+    if (any_wake_) HIP_TRY(launch_conv(d_bd_.p, d_conv_.p, nconv_, Fmax_, s));
+    if (timing_) HIP_TRY(hipEventRecord(e2, s));
+    if (any_wake_e0_) HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, ntiles_, s));
+    HIP_TRY(launch_finalize(d_bd_.p, d_list_.p, nlist_, s));
+    return HM_OK;
+}
+
+hm_status Batch::flush_timing() {
+    if (ev_used_ == 0) return HM_OK;
+    HIP_TRY(hipEventSynchronize(ev_pool_[ev_used_ - 1]));
+    for (size_t k = 0; k + 3 <= ev_used_; k += 3) {
+        float a = 0.f, b = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, ev_pool_[k], ev_pool_[k + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, ev_pool_[k + 1], ev_pool_[k + 2]));
+        pass_ms_ += a;
+        ++pass_count_;
+        if (any_wake_) {
+            conv_ms_ += b;
+            ++conv_count_;
+        }
+    }
+    ev_used_ = 0;
+    return HM_OK;
+}
+
+hm_status Batch::advance(int64_t turns) {
+    for (int64_t step = 0;; ++step) {
+        std::vector<int32_t> retiring;
+        for (size_t i = 0; i < rb_.size(); ++i)
+            if (used_[i] && rb_[i].active && rb_[i].turns_done >= rb_[i].n_turns)
+                retiring.push_back((int32_t)i);
+        if (!retiring.empty()) {
+            HM_TRY(flush_timing());
+            HM_TRY(epilogue(retiring));
+        }
+        if (step >= turns || active_count() == 0) break;
+        if (dirty_) HM_TRY(build_tables());
+        HM_TRY(one_turn());
+        for (size_t i = 0; i < rb_.size(); ++i)
+            if (used_[i] && rb_[i].active) rb_[i].turns_done++;
+    }
+    HM_TRY(flush_timing());
+    HIP_TRY(hipStreamSynchronize(ctx_->stream));
+    return HM_OK;
+}
+
+hm_status Batch::finish_now(int i) {
+    if (i < 0 || (size_t)i >= rb_.size() || !used_[(size_t)i]) return fail(HM_ERR_ARG, "bad slot");
+    if (!rb_[(size_t)i].active) return HM_OK;
+    HM_TRY(flush_timing());
+    return epilogue({(int32_t)i});
+}
+
+hm_status Batch::result(int i, hm_result *r, double *cz, double *cE) {
+    if (i < 0 || (size_t)i >= rb_.size() || !used_[(size_t)i]) return fail(HM_ERR_ARG, "bad slot");
+    RunBunch &q = rb_[(size_t)i];
+    double o[16];
+    HIP_TRY(hipMemcpy(o, q.d.out, sizeof(o), hipMemcpyDeviceToHost));
+    std::memset(r, 0, sizeof(*r));
+    r->sigma_E0 = o[0];
+    r->sigma_z0 = o[1];
+    r->sigma_E = o[2];
+    r->sigma_z = o[3];
+    r->E0 = o[4];
+    r->drift_skipped = (int32_t)o[5];
+    r->turns_done = (int64_t)o[6];
+    r->ms_pass_avg = ms_pass();
+    r->ms_conv_avg = ms_conv();
+    r->pass_launches = pass_count_;
+    r->bin_half_width = (q.d.flags & F_WAKE) ? q.d.e_last : 0.0;
+    if (cz) *cz = o[7];
+    if (cE) *cE = o[8];
+    return HM_OK;
+}
+
+void Batch::remove(int i) {
+    if (i < 0 || (size_t)i >= rb_.size()) return;
+    used_[(size_t)i] = false;
+    rb_[(size_t)i].active = false;
+    dirty_ = true;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+// ---------------------------------------------------------------------------
+// C ABI: context, bunches, evolve
+// ---------------------------------------------------------------------------
+
+static hm_status run_batch(hm_ctx *ctx, int32_t nb, hm_bunch *const *bunches, const int64_t *nbins,
+                           const hm_params *params, const hm_options *opts, hm_result *results) {
+    if (!ctx || nb <= 0 || !bunches || !params || !results) return fail(HM_ERR_ARG, "bad batch arguments");
+    HM_TRY(set_device(ctx));
+    int threads = kPassThreads;
+    bool timing = false;
+    std::vector<RunBunch> specs;
+    std::vector<int32_t> orig;
+    for (int32_t i = 0; i < nb; ++i) {
+        if (!bunches[i] || bunches[i]->ctx != ctx)
+            return fail(HM_ERR_ARG, "bunch %d is NULL or of another context", i);
+        RunBunch q;
+        q.b = bunches[i];
+        q.p = params[i];
+        if (opts) q.o = opts[i];
+        q.nbins = nbins ? nbins[i] : 0;
+        HM_TRY(validate(&q.p, &q.o, q.b->n, q.nbins));
+        if (q.o.timing) timing = true;
+        if (q.o.block_size) threads = q.o.block_size;
+        if (q.b->n == 0) {   // no wake (validated): E0 becomes 0/0 after an update (evolution.jl:426)
+            hm_result &r = results[i];
+            std::memset(&r, 0, sizeof(r));
+            r.E0 = (q.p.update_E0 && q.p.n_turns > 0) ? NAN : q.p.E0;
+            r.turns_done = q.p.n_turns;
+            continue;
+        }
+        specs.push_back(q);
+        orig.push_back(i);
+    }
+    if (specs.empty()) return HM_OK;
+    Batch B(ctx, threads);
+    B.set_timing(timing);
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, ctx->stream));
+    std::vector<int> idx;
+    HM_TRY(B.add(specs, &idx));
+    int64_t max_turns = 0;
+    for (auto &q : specs) max_turns = std::max<int64_t>(max_turns, q.p.n_turns);
+    HM_TRY(B.advance(max_turns));
+    HIP_TRY(hipEventRecord(t1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(t1));
+    float total_ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&total_ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    for (size_t k = 0; k < idx.size(); ++k) {
+        HM_TRY(B.result(idx[k], &results[orig[k]], nullptr, nullptr));
+        results[orig[k]].ms_total = total_ms;
+    }
+    return HM_OK;
+}
+
+extern "C" {
+
+const char *hm_last_error(void) { return g_last_error.c_str(); }
+
+hm_status hm_create(int32_t device, hm_ctx **out) {
+    if (!out) return fail(HM_ERR_ARG, "out is NULL");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(HM_ERR_ARG, "no HIP device %d (%d present)", device, count);
+    auto ctx = std::make_unique<hm_ctx>();
+    ctx->device = device;
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    std::snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
+    if (std::strncmp(ctx->arch, "gfx950", 6) != 0)
+        return fail(HM_ERR_DEVICE, "device %d is %s; this library is built for gfx950", device, ctx->arch);
+    ctx->cus = prop.multiProcessorCount;
+    int lds = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    ctx->lds_max = lds;
+    HIP_TRY(configure_kernels(lds));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx.release();
+    return HM_OK;
+}
+
+hm_status hm_destroy(hm_ctx *ctx) {
+    if (!ctx) return HM_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->tmp) hm_bunch_destroy(ctx->tmp);
+    for (auto &kv : ctx->twiddles) kv.second.release();
+    for (auto &b : ctx->sbuf) b.release();
+    ctx->shist.release();
+    ctx->ssc.release();
+    ctx->sbd.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HM_OK;
+}
+
+hm_status hm_device_info(hm_ctx *ctx, char *arch, int32_t arch_len, int32_t *cus, int32_t *lds) {
+    if (!ctx) return fail(HM_ERR_ARG, "ctx is NULL");
+    if (arch && arch_len > 0) std::snprintf(arch, (size_t)arch_len, "%s", ctx->arch);
+    if (cus) *cus = ctx->cus;
+    if (lds) *lds = ctx->lds_max;
+    return HM_OK;
+}
+
+hm_status hm_bunch_create(hm_ctx *ctx, int64_t n, hm_bunch **out) {
+    if (!ctx || !out) return fail(HM_ERR_ARG, "ctx/out is NULL");
+    if (n < 0) return fail(HM_ERR_ARG, "negative particle count");
+    HM_TRY(set_device(ctx));
+    auto b = std::make_unique<hm_bunch>();
+    b->ctx = ctx;
+    b->n = n;
+    const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(double);
+    hipError_t e = hipMalloc(&b->z, bytes);
+    if (e != hipSuccess) return fail(HM_ERR_OOM, "hipMalloc z (%zu bytes): %s", bytes, hipGetErrorString(e));
+    e = hipMalloc(&b->dE, bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(b->z);
+        return fail(HM_ERR_OOM, "hipMalloc dE (%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    *out = b.release();
+    return HM_OK;
+}
+
+hm_status hm_bunch_destroy(hm_bunch *b) {
+    if (!b) return HM_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(b->z);
+    (void)hipFree(b->dE);
+    b->w.release();
+    delete b;
+    return HM_OK;
+}
+
+hm_status hm_bunch_upload(hm_bunch *b, const double *z, const double *dE) {
+    if (!b || (b->n > 0 && (!z || !dE))) return fail(HM_ERR_ARG, "bad upload arguments");
+    HM_TRY(set_device(b->ctx));
+    if (b->n == 0) return HM_OK;
+    HIP_TRY(hipMemcpyAsync(b->z, z, (size_t)b->n * 8, hipMemcpyHostToDevice, b->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(b->dE, dE, (size_t)b->n * 8, hipMemcpyHostToDevice, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_bunch_download(hm_bunch *b, double *z, double *dE) {
+    if (!b || (b->n > 0 && (!z || !dE))) return fail(HM_ERR_ARG, "bad download arguments");
+    HM_TRY(set_device(b->ctx));
+    if (b->n == 0) return HM_OK;
+    HIP_TRY(hipMemcpyAsync(z, b->z, (size_t)b->n * 8, hipMemcpyDeviceToHost, b->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dE, b->dE, (size_t)b->n * 8, hipMemcpyDeviceToHost, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_bunch_device_ptrs(hm_bunch *b, double **zd, double **ed) {
+    if (!b) return fail(HM_ERR_ARG, "bunch is NULL");
+    if (zd) *zd = b->z;
+    if (ed) *ed = b->dE;
+    return HM_OK;
+}
+
+hm_status hm_bunch_generate(hm_bunch *b, uint64_t seed, double mu_z, double sigma_z, double mu_E,
+                            double sigma_E) {
+    if (!b) return fail(HM_ERR_ARG, "bunch is NULL");
+    HM_TRY(set_device(b->ctx));
+    HIP_TRY(launch_generate(seed, mu_z, sigma_z, mu_E, sigma_E, b->z, b->dE, b->n, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_bunch_evolve(hm_bunch *b, int64_t nbins, const hm_params *params,
+                          const hm_options *opts, hm_result *result) {
+    if (!b) return fail(HM_ERR_ARG, "bunch is NULL");
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    hm_bunch *arr[1] = {b};
+    return run_batch(b->ctx, 1, arr, &nbins, params, opts, result);
+}
+
+hm_status hm_batch_evolve(hm_ctx *ctx, int32_t nbunches, hm_bunch *const *bunches,
+                          const int64_t *nbins, const hm_params *params, const hm_options *opts,
+                          hm_result *results) {
+    if (!ctx) return fail(HM_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return run_batch(ctx, nbunches, bunches, nbins, params, opts, results);
+}
+
+hm_status hm_evolve(hm_ctx *ctx, double *z, double *dE, int64_t n, int64_t nbins,
+                    const hm_params *params, const hm_options *opts, hm_result *result) {
+    if (!ctx || !params || !result) return fail(HM_ERR_ARG, "ctx/params/result is NULL");
+    if (n < 0 || (n > 0 && (!z || !dE))) return fail(HM_ERR_ARG, "bad particle arrays");
+    HM_TRY(validate(params, opts, n, nbins));
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (ctx->tmp && ctx->tmp->n != n) {
+        hm_bunch_destroy(ctx->tmp);
+        ctx->tmp = nullptr;
+    }
+    if (!ctx->tmp) HM_TRY(hm_bunch_create(ctx, n, &ctx->tmp));
+    HM_TRY(hm_bunch_upload(ctx->tmp, z, dE));
+    hm_bunch *arr[1] = {ctx->tmp};
+    HM_TRY(run_batch(ctx, 1, arr, &nbins, params, opts, result));
+    return hm_bunch_download(ctx->tmp, z, dE);
+}
+
+hm_status hm_bin_grid(double zw, int64_t nbins, double *edges, double *centers, double *step) {
+    if (nbins <= 0 || !(zw > 0)) return fail(HM_ERR_ARG, "bad arguments");
+    bin_grid(zw, nbins, edges, centers, step);
+    return HM_OK;
+}
+
+}  // extern "C"

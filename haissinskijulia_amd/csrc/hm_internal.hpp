@@ -1,0 +1,179 @@
+// hm_internal.hpp -- host-side internals shared by the engine, the stage
+// entry points and the server (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/haissinski_mi355x.h"
+#include "hm_kernels.hpp"
+#include "hm_types.hpp"
+
+namespace hm {
+
+hm_status fail(hm_status st, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return ::hm::fail(e_ == hipErrorOutOfMemory ? HM_ERR_OOM : HM_ERR_DEVICE,        \
+                              "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                              __LINE__);                                                     \
+    } while (0)
+
+#define HM_TRY(expr)                \
+    do {                            \
+        hm_status s_ = (expr);      \
+        if (s_ != HM_OK) return s_; \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;  // elements
+    hm_status ensure(size_t n) {
+        if (n <= cap && p) return HM_OK;
+        release();
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(HM_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        }
+        cap = std::max<size_t>(n, 1);
+        return HM_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    size_t bytes() const { return cap * sizeof(T); }
+};
+
+// per-bunch work buffers of an evolve call (kept between calls)
+struct Work {
+    DevBuf<uint32_t> hist;
+    DevBuf<double> P, lam, What, Wtmp, part, ipart, moments, normals, out;
+    DevBuf<TurnScalars> sc;
+    void release() {
+        hist.release(); P.release(); lam.release(); What.release(); Wtmp.release();
+        part.release(); ipart.release(); moments.release(); normals.release(); out.release();
+        sc.release();
+    }
+    size_t bytes() const {
+        return hist.bytes() + P.bytes() + lam.bytes() + What.bytes() + Wtmp.bytes() +
+               part.bytes() + ipart.bytes() + moments.bytes() + normals.bytes() + out.bytes() +
+               sc.bytes();
+    }
+};
+
+}  // namespace hm
+
+struct hm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    int lds_max = 65536;
+    char arch[64] = {0};
+    std::map<int, hm::DevBuf<double>> twiddles;   // F -> exp(-2 pi i k/F), k < F/2
+    hm_bunch *tmp = nullptr;                      // staging bunch of hm_evolve
+    hm::DevBuf<double> sbuf[4];                   // stage-entry scratch
+    hm::DevBuf<uint32_t> shist;
+    hm::DevBuf<hm::TurnScalars> ssc;
+    hm::DevBuf<hm::BunchDev> sbd;
+    std::recursive_mutex mu;
+};
+
+struct hm_bunch {
+    hm_ctx *ctx = nullptr;
+    int64_t n = 0;
+    double *z = nullptr, *dE = nullptr;
+    hm::Work w;
+};
+
+namespace hm {
+
+hm_status set_device(hm_ctx *ctx);
+double rn_ratio(double x, int64_t num, int64_t den);
+void bin_grid(double zw, int64_t nb, double *edges, double *centers, double *step);
+double particle_factor(int64_t N);
+int ilog2(int64_t x);
+hm_status twiddles(hm_ctx *ctx, int F, const double **out);
+hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t nbins);
+int max_fft_block(const hm_ctx *ctx);
+int auto_fft_block(int64_t nw, int64_t nbins, int max_block);
+
+// One bunch inside a Batch.
+struct RunBunch {
+    hm_bunch *b = nullptr;
+    hm_params p{};
+    hm_options o{};
+    std::vector<double> wake_copy, res_copy;   // wake table / resonators until set-up
+    int64_t nbins = 0;
+    int64_t n_turns = 0;      // turns to run
+    int64_t turns_done = 0;
+    BunchDev d{};
+    bool active = false;      // still turning
+    bool finished = false;    // epilogue done, results valid
+};
+
+// A set of device-resident bunches that advance together: every turn is one
+// turn-pass launch over the tiles of all active bunches, one convolution
+// launch over their output blocks, [one interpolation-sum launch] and one
+// finalize launch.  Bunches may join (add) and leave (remove) between turns.
+class Batch {
+  public:
+    explicit Batch(hm_ctx *ctx, int threads = kPassThreads) : ctx_(ctx), threads_(threads) {}
+    ~Batch();
+    // set up new bunches (initial spreads, grids, tables, FFT plan); their
+    // slot indices are appended to idx
+    hm_status add(const std::vector<RunBunch> &specs, std::vector<int> *idx);
+    // run up to `turns` turns of every active bunch; bunches that reach their
+    // n_turns run the epilogue and become finished
+    hm_status advance(int64_t turns);
+    // stop a bunch now (epilogue included), e.g. a failed or cancelled job
+    hm_status finish_now(int i);
+    hm_status result(int i, hm_result *r, double *centroid_z, double *centroid_E);
+    void remove(int i);   // forget a finished slot (the bunch itself is the caller's)
+    int active_count() const;
+    size_t size() const { return rb_.size(); }
+    RunBunch &slot(int i) { return rb_[(size_t)i]; }
+    void set_timing(bool t) { timing_ = t; }
+    double ms_pass() const { return pass_count_ ? pass_ms_ / (double)pass_count_ : 0.0; }
+    double ms_conv() const { return conv_count_ ? conv_ms_ / (double)conv_count_ : 0.0; }
+    int64_t pass_count() const { return pass_count_; }
+
+  private:
+    hm_status build_tables();
+    hm_status upload_descriptors();
+    hm_status run_subset_moments(const std::vector<int32_t> &which, int mode);
+    hm_status epilogue(const std::vector<int32_t> &which);
+    hm_status one_turn();
+    hm_status flush_timing();
+
+    hm_ctx *ctx_;
+    int threads_;
+    std::vector<RunBunch> rb_;
+    std::vector<bool> used_;
+    DevBuf<BunchDev> d_bd_;
+    DevBuf<int32_t> d_tiles_, d_conv_, d_list_, d_tmp_tiles_, d_tmp_list_;
+    int ntiles_ = 0, nconv_ = 0, nlist_ = 0;
+    size_t pass_lds_ = 0;
+    int Fmax_ = 2;
+    bool any_wake_ = false, any_wake_e0_ = false;
+    bool dirty_ = true;
+    bool timing_ = false;
+    std::vector<hipEvent_t> ev_pool_;
+    size_t ev_used_ = 0;
+    double pass_ms_ = 0.0, conv_ms_ = 0.0;
+    int64_t pass_count_ = 0, conv_count_ = 0;
+};
+
+}  // namespace hm

@@ -1,0 +1,865 @@
+// hm_kernels.hip -- gfx950 kernels of the longitudinal turn loop.
+//
+// One turn of the reference (src/evolution.jl:295-456) is, per bunch:
+//   k_turn_pass   one coalesced read and write of (z, dE) per particle:
+//                 the previous turn's wake kick and E0 re-centre, then RF kick
+//                 (rf_kick.jl:84-87), quantum excitation (quantum_excitation.jl
+//                 :58-71), SR damping (synchrotron_radiation.jl:44-56) and the
+//                 drift (evolution.jl:330-347); it bins the new z into an
+//                 LDS-privatised histogram (utils.jl:351-401) and writes
+//                 deterministic per-tile partial sums.
+//   k_conv        the bin phase: sigma_z, the wake current (evolution.jl:373-386)
+//                 and the line-density convolution with the wake
+//                 (wakefield.jl:186-319) by a blocked LDS FFT.
+//   k_interp_sum  (update_E0 with the wake) sum of the interpolated potential
+//                 over the post-drift z, which the E0 update of
+//                 evolution.jl:404-435 needs before the next turn.
+//   k_finalize    E0 update, scalars of the next turn, histogram reset.
+//
+// All floating-point work is FP64 with no contraction (-ffp-contract=off);
+// sums are reduced in a fixed order, so a run is bitwise reproducible.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "hm_kernels.hpp"
+
+namespace hm {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline double jl_max(double a, double b) {
+    // Julia's max for floats: NaN wins
+    if (a != a) return a;
+    if (b != b) return b;
+    return (b > a) ? b : a;
+}
+
+// compute_global_std from reduced sums (utils.jl:589-605)
+__device__ inline double gstd_from_sums(double s, double s2, double nd) {
+    const int64_t n = (int64_t)nd;
+    if (n <= 1) return 0.0;
+    const double mean = s / (double)n;
+    const double var = (s2 / (double)n) - (mean * mean);
+    return sqrt(jl_max(0.0, var * (double)n / (double)(n - 1)));
+}
+
+// Scalars of a turn from the current E0 (evolution.jl:300-304,157;
+// synchrotron_radiation.jl:44-45; quantum_excitation.jl:58-59).
+__device__ inline void turn_scalars(const BunchDev &B, double E0, TurnScalars &s) {
+    s.E0 = E0;
+    s.gamma0 = E0 / B.mass;
+    const double ig2 = 1.0 / (s.gamma0 * s.gamma0);
+    s.beta0 = sqrt(jl_max(0.0, 1.0 - ig2));
+    s.eta0 = B.alpha_c - ig2;
+    s.rf = (s.beta0 <= 0.0) ? INFINITY : (B.freq_rf * kTwoPi) / (s.beta0 * kSpeedLight);
+    s.denom = (s.beta0 * s.beta0) * E0;
+    s.coeff_c = (B.two_pi_h * s.eta0) / s.denom;
+    const double x = E0 / 1e9;
+    const double d = ((4.0 * 8.85e-5) * ((x * x) * x)) / B.radius;
+    s.damp = 1.0 - d;
+    const double om = 1.0 - d;
+    s.exc = sqrt(1.0 - (om * om)) * B.sigma_E0;
+    s.drift_skip = (!(B.flags & F_ETA)) && (s.beta0 <= 0.0 || E0 == 0.0 || s.rf == 0.0);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11)
+__device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                     uint32_t k0, uint32_t k1, uint32_t x[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    x[0] = c0; x[1] = c1; x[2] = c2; x[3] = c3;
+}
+
+// Box-Muller (1958) on one Philox block: u1 in (0,1], u2 in [0,1), 53 bits each
+__device__ inline void box_muller(const uint32_t x[4], double &n0, double &n1) {
+    const double u1 = ((double)(x[0] >> 5) * 67108864.0 + (double)(x[1] >> 6) + 1.0) * 0x1p-53;
+    const double u2 = ((double)(x[2] >> 5) * 67108864.0 + (double)(x[3] >> 6)) * 0x1p-53;
+    const double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincos(kTwoPi * u2, &s, &c);
+    n0 = r * c;
+    n1 = r * s;
+}
+
+// QE normals of particle pair p at absolute turn t
+__device__ inline void qe_pair_normals(uint64_t seed, uint64_t p, uint32_t t, double &n0,
+                                       double &n1) {
+    uint32_t x[4];
+    philox4x32_10((uint32_t)p, (uint32_t)(p >> 32), t, 0u, (uint32_t)seed,
+                  (uint32_t)(seed >> 32), x);
+    box_muller(x, n0, n1);
+}
+
+// zero_alloc_interpolation! for one particle (wakefield.jl:32-36,38-42,64-78,94-102)
+__device__ inline double interp_pot(const double *__restrict__ P, double z, double c0, double c1,
+                                    double inv, int nb) {
+    if (nb <= 1) return nb == 1 ? P[0] : 0.0;
+    const double cl = (z > c1) ? c1 : ((z < c0) ? c0 : z);
+    const double u = (cl - c0) * inv;
+    const double fu = floor(u);
+    int k = (fu == fu) ? (int)fu : 0;
+    k = k < 0 ? 0 : (k > nb - 2 ? nb - 2 : k);
+    const double w = u - fu;
+    return P[k] * (1.0 - w) + P[k + 1] * w;
+}
+
+// histogram bin of calculate_local_histogram! (utils.jl:387-397); -1 = not counted
+__device__ inline int hist_bin(double z, double e0, double e1, double inv, int nb) {
+    if (!((z >= e0) & (z <= e1))) return -1;
+    const double u = (z - e0) * inv;
+    int b = (int)floor(u);
+    return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+}
+
+// Deterministic block sum of NV values: wave butterflies, then the waves in
+// order.  The result is valid in thread 0.  s_red holds NV * 16 doubles.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double *s_red) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s_red[k * 16 + wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += s_red[k * 16 + w];
+            v[k] = s;
+        }
+    }
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline int bitrev(int m, int logF) { return (int)(__brev((unsigned)m) >> (32 - logF)); }
+
+// In-place radix-2 decimation-in-time FFT of X[F] in LDS, bit-reversed input,
+// natural-order output, twiddles tw[k] = exp(-2 pi i k / F).
+__device__ void fft_dit_forward(double2 *X, int F, int logF, const double2 *__restrict__ tw) {
+    for (int s = 0; s < logF; ++s) {
+        const int h = 1 << s, tsh = logF - 1 - s;
+        __syncthreads();
+        for (int b = threadIdx.x; b < (F >> 1); b += blockDim.x) {
+            const int k = b & (h - 1);
+            const int i = ((b >> s) << (s + 1)) | k;
+            const double2 w = tw[k << tsh];
+            const double2 a = X[i];
+            const double2 t = cmul(X[i + h], w);
+            X[i] = make_double2(a.x + t.x, a.y + t.y);
+            X[i + h] = make_double2(a.x - t.x, a.y - t.y);
+        }
+    }
+    __syncthreads();
+}
+
+// Unnormalised inverse FFT (decimation in frequency, conjugate twiddles):
+// natural-order input, output y[m] at X[bitrev(m)].
+__device__ void fft_dif_inverse(double2 *X, int F, int logF, const double2 *__restrict__ tw) {
+    for (int s = logF - 1; s >= 0; --s) {
+        const int h = 1 << s, tsh = logF - 1 - s;
+        __syncthreads();
+        for (int b = threadIdx.x; b < (F >> 1); b += blockDim.x) {
+            const int k = b & (h - 1);
+            const int i = ((b >> s) << (s + 1)) | k;
+            const double2 t = tw[k << tsh];
+            const double2 w = make_double2(t.x, -t.y);
+            const double2 a = X[i], c = X[i + h];
+            X[i] = make_double2(a.x + c.x, a.y + c.y);
+            X[i + h] = cmul(make_double2(a.x - c.x, a.y - c.y), w);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// per-particle turn (fused)
+// ---------------------------------------------------------------------------
+
+struct PassCtx {
+    // current turn
+    double rf, phis, sin_phis, voltage, exc, damp, mass, gamma0, alpha_c, two_pi_h, denom,
+        coeff_c, m_prev;
+    int pending_kick, pending_m, drift_skip;
+    uint32_t flags;
+    // grids
+    const double *P;
+    double c0, c1, inv_s, e0, e1, inv_step;
+    int nb;
+};
+
+__device__ inline void load_ctx(const BunchDev &B, PassCtx &c) {
+    const TurnScalars &s = *B.sc;
+    c.rf = s.rf;
+    c.phis = B.phis;
+    c.sin_phis = B.sin_phis;
+    c.voltage = B.voltage;
+    c.exc = s.exc;
+    c.damp = s.damp;
+    c.mass = B.mass;
+    c.gamma0 = s.gamma0;
+    c.alpha_c = B.alpha_c;
+    c.two_pi_h = B.two_pi_h;
+    c.denom = s.denom;
+    c.coeff_c = s.coeff_c;
+    c.m_prev = s.m_prev;
+    c.pending_kick = s.pending_kick;
+    c.pending_m = s.pending_m;
+    c.drift_skip = s.drift_skip;
+    c.flags = B.flags;
+    c.P = B.P;
+    c.c0 = B.c_first;
+    c.c1 = B.c_last;
+    c.inv_s = B.inv_s;
+    c.e0 = B.e_first;
+    c.e1 = B.e_last;
+    c.inv_step = B.inv_step;
+    c.nb = B.nbins;
+}
+
+// The previous turn's wake kick (wakefield.jl:342-344) and E0 re-centre
+// (evolution.jl:434), applied at the start of this pass.
+__device__ inline double apply_pending(const PassCtx &c, double z, double e) {
+    if (c.pending_kick) e = e - interp_pot(c.P, z, c.c0, c.c1, c.inv_s, c.nb);
+    if (c.pending_m) e = e - c.m_prev;
+    return e;
+}
+
+// RF, QE, SR and drift of one particle; returns dE' - dE_init via de_turn.
+__device__ inline void advance(const PassCtx &c, double &z, double &e, double r, double &de_turn) {
+    const double e_init = e;
+    const double phi0 = ((-z) * c.rf) + c.phis;                    // rf_kick.jl:85
+    e = e + c.voltage * (sin(phi0) - c.sin_phis);                  // rf_kick.jl:86
+    if (c.flags & F_QE) e = e + c.exc * r;                         // quantum_excitation.jl:70
+    if (c.flags & F_SR) e = e * c.damp;                            // synchrotron_radiation.jl:55
+    if (c.flags & F_ETA) {                                         // evolution.jl:332-343
+        const double dg = e / c.mass;
+        const double gp = c.gamma0 + dg;
+        const double eta = c.alpha_c - 1.0 / (gp * gp);
+        const double coeff = (c.two_pi_h * eta) / c.denom;
+        double phi = -((z * c.rf) - c.phis);
+        phi = phi + coeff * e;
+        z = ((-phi) + c.phis) / c.rf;
+    } else if (!c.drift_skip) {                                    // evolution.jl:159-165
+        double phi = -((z * c.rf) - c.phis);
+        phi = phi + c.coeff_c * e;
+        z = ((-phi) + c.phis) / c.rf;
+    }
+    de_turn = e - e_init;
+}
+
+__global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__restrict__ bds,
+                                                            const int32_t *__restrict__ tile_bunch) {
+    extern __shared__ uint32_t s_hist[];
+    __shared__ double s_red[4 * 16];
+    const int tile = blockIdx.x;
+    const BunchDev &B = bds[tile_bunch[tile]];
+    const int lt = tile - B.tile_begin;
+    PassCtx c;
+    load_ctx(B, c);
+    const bool wake = c.flags & F_WAKE;
+    const bool lds_hist = wake && !(c.flags & F_GHIST);
+    uint32_t *hist = lds_hist ? s_hist : B.hist;
+    if (lds_hist) {
+        for (int b = threadIdx.x; b < c.nb; b += blockDim.x) s_hist[b] = 0u;
+        __syncthreads();
+    }
+    const int64_t n = B.n;
+    const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
+    const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
+    const TurnScalars &sc = *B.sc;
+    const uint32_t tabs = (uint32_t)(B.turn_offset + sc.turn);
+    const double *nrm = (c.flags & F_INJ) ? B.normals + sc.turn * n : nullptr;
+    double *__restrict__ zp = B.z;
+    double *__restrict__ ep = B.dE;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const int64_t i = 2 * p;
+        const bool two = (i + 1 < n);
+        double z0, z1 = 0.0, e0, e1 = 0.0;
+        if (two) {
+            const double2 zz = *reinterpret_cast<const double2 *>(zp + i);
+            const double2 ee = *reinterpret_cast<const double2 *>(ep + i);
+            z0 = zz.x; z1 = zz.y; e0 = ee.x; e1 = ee.y;
+        } else {
+            z0 = zp[i];
+            e0 = ep[i];
+        }
+        double r0 = 0.0, r1 = 0.0;
+        if (c.flags & F_QE) {
+            if (nrm) {
+                r0 = nrm[i];
+                if (two) r1 = nrm[i + 1];
+            } else {
+                qe_pair_normals(B.seed, (uint64_t)p, tabs, r0, r1);
+            }
+        }
+        double d0, d1 = 0.0;
+        e0 = apply_pending(c, z0, e0);
+        advance(c, z0, e0, r0, d0);
+        if (two) {
+            e1 = apply_pending(c, z1, e1);
+            advance(c, z1, e1, r1, d1);
+        }
+        if (wake) {
+            v[0] += z0;
+            v[1] += z0 * z0;
+            const int b0 = hist_bin(z0, c.e0, c.e1, c.inv_step, c.nb);
+            if (b0 >= 0) { atomicAdd(&hist[b0], 1u); v[3] += 1.0; }
+            if (two) {
+                v[0] += z1;
+                v[1] += z1 * z1;
+                const int b1 = hist_bin(z1, c.e0, c.e1, c.inv_step, c.nb);
+                if (b1 >= 0) { atomicAdd(&hist[b1], 1u); v[3] += 1.0; }
+            }
+        }
+        v[2] += d0;
+        if (two) v[2] += d1;
+        if (two) {
+            *reinterpret_cast<double2 *>(zp + i) = make_double2(z0, z1);
+            *reinterpret_cast<double2 *>(ep + i) = make_double2(e0, e1);
+        } else {
+            zp[i] = z0;
+            ep[i] = e0;
+        }
+    }
+    block_sum<4>(v, s_red);
+    if (threadIdx.x == 0) {
+        double *o = B.part + 4 * (int64_t)lt;
+        o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    }
+    if (lds_hist) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < c.nb; b += blockDim.x) {
+            const uint32_t h = s_hist[b];
+            if (h) atomicAdd(&B.hist[b], h);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bin phase: sigma_z, current, line density, blocked FFT convolution
+// ---------------------------------------------------------------------------
+//
+// P_k = I * sum_{j<=k} W_j g_{k-j},  g_i = lam_i * (cnt_i / n_h).
+// With block size B and F = 2B, output block o is
+//   Re(IFFT(Y))[0,B) + Im(IFFT(Y))[B,2B),  Y = sum_j FFT(g_{o-j} + i g_{o-1-j}) * FFT(W_j),
+// since each block product g_a * W_j is real, has length 2B-1 <= F (no
+// wrap), and lands on output blocks a+j (first half) and a+j+1 (second half).
+
+__global__ __launch_bounds__(kConvThreads) void k_conv(const BunchDev *__restrict__ bds,
+                                                       const int32_t *__restrict__ items) {
+    extern __shared__ double2 sX[];
+    __shared__ double s_red[3 * 16];
+    __shared__ double s_bc[3];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int o = items[2 * blockIdx.x + 1];
+    const TurnScalars &sc = *B.sc;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
+        v[0] += B.part[4 * t];
+        v[1] += B.part[4 * t + 1];
+        v[2] += B.part[4 * t + 3];
+    }
+    block_sum<3>(v, s_red);
+    if (threadIdx.x == 0) {
+        const double sz = gstd_from_sums(v[0], v[1], (double)B.n);   // evolution.jl:357
+        const double cur = (B.flags & F_CHARGE)
+                               ? B.charge
+                               : (((B.pf / sc.E0) / (kTwoPi * B.radius)) * sz) /
+                                     (sc.eta0 * (B.sigma_E0 * B.sigma_E0));  // evolution.jl:386
+        s_bc[0] = sz;
+        s_bc[1] = cur;
+        s_bc[2] = v[2];
+        if (o == 0) {
+            B.sc->sigma_z = sz;
+            B.sc->current = cur;
+            B.sc->n_h = v[2];
+        }
+    }
+    __syncthreads();
+    const double cur = s_bc[1], nh = s_bc[2];
+    const int nb = B.nbins, Bk = B.B, F = B.F, logF = B.logF, nJ = B.nJ, nA = B.nA;
+    if (nh == 0.0) {  // wakefield.jl:305-311: no particles binned -> zero potential
+        for (int m = threadIdx.x; m < Bk; m += blockDim.x)
+            if (o * Bk + m < nb) B.P[o * Bk + m] = 0.0;
+        return;
+    }
+    const double inv_nh = 1.0 / nh;                                 // wakefield.jl:207
+    const double2 *tw = reinterpret_cast<const double2 *>(B.tw);
+    const double2 *What = reinterpret_cast<const double2 *>(B.What);
+    double2 acc[kConvMaxPerThread];
+#pragma unroll
+    for (int q = 0; q < kConvMaxPerThread; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int j = 0; j < nJ; ++j) {
+        const int a = o - j, bb = o - 1 - j;
+        for (int m = threadIdx.x; m < F; m += blockDim.x) {
+            double re = 0.0, im = 0.0;
+            if (m < Bk) {
+                if (a >= 0 && a < nA) {
+                    const int i = a * Bk + m;
+                    if (i < nb) re = B.lam[i] * ((double)B.hist[i] * inv_nh);  // wakefield.jl:210,238
+                }
+                if (bb >= 0 && bb < nA) {
+                    const int i = bb * Bk + m;
+                    if (i < nb) im = B.lam[i] * ((double)B.hist[i] * inv_nh);
+                }
+            }
+            sX[bitrev(m, logF)] = make_double2(re, im);
+        }
+        fft_dit_forward(sX, F, logF, tw);
+#pragma unroll
+        for (int q = 0; q < kConvMaxPerThread; ++q) {
+            const int k = threadIdx.x + q * kConvThreads;
+            if (k < F) {
+                const double2 t = cmul(sX[k], What[(int64_t)j * F + k]);
+                acc[q] = make_double2(acc[q].x + t.x, acc[q].y + t.y);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kConvMaxPerThread; ++q) {
+        const int k = threadIdx.x + q * kConvThreads;
+        if (k < F) sX[k] = acc[q];
+    }
+    fft_dif_inverse(sX, F, logF, tw);
+    const double scale = cur / (double)F;
+    for (int m = threadIdx.x; m < Bk; m += blockDim.x) {
+        const int i = o * Bk + m;
+        if (i >= nb) continue;
+        const double2 y0 = sX[bitrev(m, logF)];
+        const double2 y1 = sX[bitrev(m + Bk, logF)];
+        B.P[i] = (y0.x + y1.y) * scale;
+    }
+}
+
+// FFT of one wake block per workgroup (set-up of an evolve call)
+__global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__restrict__ bd,
+                                                           const double *__restrict__ W) {
+    extern __shared__ double2 sX[];
+    const BunchDev &B = *bd;
+    const int j = blockIdx.x, Bk = B.B, F = B.F, logF = B.logF, nb = B.nbins;
+    for (int m = threadIdx.x; m < F; m += blockDim.x) {
+        const int i = j * Bk + m;
+        const double w = (m < Bk && i < nb) ? W[i] : 0.0;
+        sX[bitrev(m, logF)] = make_double2(w, 0.0);
+    }
+    fft_dit_forward(sX, F, logF, reinterpret_cast<const double2 *>(B.tw));
+    double2 *out = reinterpret_cast<double2 *>(const_cast<double *>(B.What)) + (int64_t)j * F;
+    for (int k = threadIdx.x; k < F; k += blockDim.x) out[k] = sX[k];
+}
+
+// ---------------------------------------------------------------------------
+// E0 update support, moments, init, finalize
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kPassThreads) void k_interp_sum(const BunchDev *__restrict__ bds,
+                                                             const int32_t *__restrict__ tile_bunch) {
+    __shared__ double s_red[16];
+    const int tile = blockIdx.x;
+    const BunchDev &B = bds[tile_bunch[tile]];
+    if (!((B.flags & F_WAKE) && (B.flags & F_E0))) return;
+    const int lt = tile - B.tile_begin;
+    const int64_t i0 = 2 * (int64_t)lt * B.pairs_per_tile;
+    const int64_t i1 = min(i0 + 2 * B.pairs_per_tile, B.n);
+    const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
+    const int nb = B.nbins;
+    double v[1] = {0.0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+        v[0] += interp_pot(B.P, B.z[i], c0, c1, inv, nb);
+    block_sum<1>(v, s_red);
+    if (threadIdx.x == 0) B.ipart[lt] = v[0];
+}
+
+__global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__restrict__ bds,
+                                                          const int32_t *__restrict__ tile_bunch,
+                                                          int mode) {
+    __shared__ double s_red[4 * 16];
+    const int tile = blockIdx.x;
+    const BunchDev &B = bds[tile_bunch[tile]];
+    const int lt = tile - B.tile_begin;
+    const int64_t i0 = 2 * (int64_t)lt * B.pairs_per_tile;
+    const int64_t i1 = min(i0 + 2 * B.pairs_per_tile, B.n);
+    PassCtx c;
+    if (mode == 1) load_ctx(B, c);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const double z = B.z[i];
+        double e = B.dE[i];
+        if (mode == 1 && (c.pending_kick || c.pending_m)) {
+            e = apply_pending(c, z, e);
+            B.dE[i] = e;
+        }
+        v[0] += z;
+        v[1] += z * z;
+        v[2] += e;
+        v[3] += e * e;
+    }
+    block_sum<4>(v, s_red);
+    if (threadIdx.x == 0) {
+        double *o = B.moments + 4 * (int64_t)lt;
+        o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    }
+}
+
+__global__ __launch_bounds__(kFinalThreads) void k_moments_final(const BunchDev *__restrict__ bds,
+                                                                 const int32_t *__restrict__ list,
+                                                                 int mode) {
+    __shared__ double s_red[4 * 16];
+    const BunchDev &B = bds[list[blockIdx.x]];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
+        v[0] += B.moments[4 * t];
+        v[1] += B.moments[4 * t + 1];
+        v[2] += B.moments[4 * t + 2];
+        v[3] += B.moments[4 * t + 3];
+    }
+    block_sum<4>(v, s_red);
+    if (threadIdx.x == 0) {
+        const double sE = gstd_from_sums(v[2], v[3], (double)B.n);
+        const double sz = gstd_from_sums(v[0], v[1], (double)B.n);
+        if (mode == 0) {
+            B.out[0] = sE;
+            B.out[1] = sz;
+        } else {
+            const TurnScalars &s = *B.sc;
+            B.out[2] = sE;
+            B.out[3] = sz;
+            B.out[4] = s.E0;
+            B.out[5] = (double)s.drift_skipped_count;
+            B.out[6] = (double)s.turn;
+            B.out[7] = (B.n > 0) ? v[0] / (double)B.n : 0.0;   // centroid z
+            B.out[8] = (B.n > 0) ? v[2] / (double)B.n : 0.0;   // centroid dE
+        }
+    }
+}
+
+__global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restrict__ bds,
+                                                        const int32_t *__restrict__ list) {
+    const BunchDev &B = bds[list[blockIdx.x]];
+    if (B.flags & F_WAKE)
+        for (int b = threadIdx.x; b < B.nbins; b += blockDim.x) B.hist[b] = 0u;
+    if (threadIdx.x == 0) {
+        TurnScalars s = {};
+        turn_scalars(B, B.E0_init, s);
+        s.m_prev = 0.0;
+        s.pending_kick = 0;
+        s.pending_m = 0;
+        s.drift_skipped_count = 0;
+        s.turn = 0;
+        *B.sc = s;
+    }
+}
+
+__global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__restrict__ bds,
+                                                            const int32_t *__restrict__ list) {
+    __shared__ double s_red[2 * 16];
+    const BunchDev &B = bds[list[blockIdx.x]];
+    const uint32_t fl = B.flags;
+    double v[2] = {0.0, 0.0};
+    if (fl & F_E0) {
+        for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
+            v[0] += B.part[4 * t + 2];
+            if (fl & F_WAKE) v[1] += B.ipart[t];
+        }
+    }
+    block_sum<2>(v, s_red);
+    if (fl & F_WAKE)
+        for (int b = threadIdx.x; b < B.nbins; b += blockDim.x) B.hist[b] = 0u;
+    if (threadIdx.x == 0) {
+        TurnScalars s = *B.sc;
+        double E0 = s.E0;
+        if (s.drift_skip) s.drift_skipped_count += 1;
+        if (fl & F_E0) {                                  // evolution.jl:419-429
+            const double mean = (v[0] - v[1]) / (double)B.n;
+            E0 = E0 + mean;
+            s.m_prev = mean;
+            s.pending_m = 1;
+        } else {
+            s.pending_m = 0;
+        }
+        s.pending_kick = (fl & F_WAKE) ? 1 : 0;
+        s.turn += 1;
+        turn_scalars(B, E0, s);
+        *B.sc = s;
+    }
+}
+
+__global__ void k_turn_scalars(const BunchDev *__restrict__ bd, double E0, TurnScalars *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        TurnScalars s = {};
+        turn_scalars(*bd, E0, s);
+        *out = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stage kernels (unit parity)
+// ---------------------------------------------------------------------------
+
+__global__ void k_stage_rf(double V, double sphis, double rf, double phis,
+                           const double *__restrict__ z, double *__restrict__ dE, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double phi = ((-z[i]) * rf) + phis;
+        dE[i] = dE[i] + V * (sin(phi) - sphis);
+    }
+}
+
+__global__ void k_stage_qe(double exc, const double *__restrict__ r, double *__restrict__ dE,
+                           int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dE[i] = dE[i] + exc * r[i];
+}
+
+__global__ void k_stage_sr(double damp, double *__restrict__ dE, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dE[i] = dE[i] * damp;
+}
+
+__global__ void k_stage_drift(int update_eta, const TurnScalars *__restrict__ scp, double mass,
+                              double alpha_c, double two_pi_h, double phis,
+                              double *__restrict__ z, const double *__restrict__ dE, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const TurnScalars &s = *scp;
+    PassCtx c = {};
+    c.rf = s.rf;
+    c.phis = phis;
+    c.mass = mass;
+    c.gamma0 = s.gamma0;
+    c.alpha_c = alpha_c;
+    c.two_pi_h = two_pi_h;
+    c.denom = s.denom;
+    c.coeff_c = s.coeff_c;
+    c.drift_skip = s.drift_skip;
+    c.flags = update_eta ? F_ETA : 0u;
+    double zz = z[i];
+    const double e = dE[i];
+    if (c.flags & F_ETA) {
+        const double dg = e / c.mass;
+        const double gp = c.gamma0 + dg;
+        const double eta = c.alpha_c - 1.0 / (gp * gp);
+        const double coeff = (c.two_pi_h * eta) / c.denom;
+        double phi = -((zz * c.rf) - c.phis);
+        phi = phi + coeff * e;
+        zz = ((-phi) + c.phis) / c.rf;
+    } else if (!c.drift_skip) {
+        double phi = -((zz * c.rf) - c.phis);
+        phi = phi + c.coeff_c * e;
+        zz = ((-phi) + c.phis) / c.rf;
+    }
+    z[i] = zz;
+}
+
+__global__ __launch_bounds__(kPassThreads) void k_stage_hist(const double *__restrict__ z,
+                                                             int64_t n, double e0, double e1,
+                                                             double inv, int nb,
+                                                             uint32_t *__restrict__ ghist,
+                                                             int use_lds) {
+    extern __shared__ uint32_t s_hist[];
+    uint32_t *h = use_lds ? s_hist : ghist;
+    if (use_lds) {
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) s_hist[b] = 0u;
+        __syncthreads();
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = hist_bin(z[i], e0, e1, inv, nb);
+        if (b >= 0) atomicAdd(&h[b], 1u);
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < nb; b += blockDim.x)
+            if (s_hist[b]) atomicAdd(&ghist[b], s_hist[b]);
+    }
+}
+
+__global__ void k_stage_interp_kick(const double *__restrict__ z, double *__restrict__ dE,
+                                    int64_t n, const double *__restrict__ P, double c0, double c1,
+                                    double inv, int nb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dE[i] -= interp_pot(P, z[i], c0, c1, inv, nb);
+}
+
+__global__ void k_stage_normals(uint64_t seed, int64_t turn, int64_t i0, int64_t n,
+                                double *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = i0 + k;
+    double a, b;
+    qe_pair_normals(seed, (uint64_t)i >> 1, (uint32_t)turn, a, b);
+    out[k] = (i & 1) ? b : a;
+}
+
+__global__ void k_generate(uint64_t seed, double mu_z, double sz, double mu_E, double sE,
+                           double *__restrict__ z, double *__restrict__ dE, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x[4];
+    philox4x32_10((uint32_t)i, (uint32_t)((uint64_t)i >> 32), 0u, 1u, (uint32_t)seed,
+                  (uint32_t)(seed >> 32), x);
+    double a, b;
+    box_muller(x, a, b);
+    z[i] = mu_z + sz * a;
+    dE[i] = mu_E + sE * b;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+
+static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t configure_kernels(int lds) {
+    hipError_t e;
+    e = hipFuncSetAttribute((const void *)k_turn_pass, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_conv, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_fft_wake, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_stage_hist, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+hipError_t launch_moments(const BunchDev *bds, const int32_t *tb, int ntiles, int mode,
+                          hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_moments, dim3(ntiles), dim3(kPassThreads), 0, s, bds, tb, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments_final(const BunchDev *bds, const int32_t *list, int nb, int mode,
+                                hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_moments_final, dim3(nb), dim3(kFinalThreads), 0, s, bds, list, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_init(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init, dim3(nb), dim3(kFinalThreads), 0, s, bds, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, int ntiles, size_t lds,
+                            int threads, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_turn_pass, dim3(ntiles), dim3(threads), lds, s, bds, tb);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv(const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
+                       hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_conv, dim3(nitems), dim3(kConvThreads), (size_t)Fmax * 16, s, bds, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_interp_sum, dim3(ntiles), dim3(kPassThreads), 0, s, bds, tb);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(kFinalThreads), 0, s, bds, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_wake(const BunchDev *bd, const double *W, int nJ, int F, hipStream_t s) {
+    if (nJ <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fft_wake, dim3(nJ), dim3(kConvThreads), (size_t)F * 16, s, bd, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_rf(double V, double sphis, double rf, double phis, const double *z,
+                           double *dE, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_rf, dim3(blocks_for(n, 256)), dim3(256), 0, s, V, sphis, rf, phis,
+                       z, dE, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_qe(double exc, const double *r, double *dE, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_qe, dim3(blocks_for(n, 256)), dim3(256), 0, s, exc, r, dE, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_sr(double damp, double *dE, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_sr, dim3(blocks_for(n, 256)), dim3(256), 0, s, damp, dE, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_drift(int update_eta, const TurnScalars *sc, double mass, double alpha_c,
+                              double two_pi_h, double phis, double *z, const double *dE,
+                              int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_drift, dim3(blocks_for(n, 256)), dim3(256), 0, s, update_eta, sc,
+                       mass, alpha_c, two_pi_h, phis, z, dE, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_hist(const double *z, int64_t n, double e0, double e1, double inv, int nb,
+                             uint32_t *hist, int use_lds, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = blocks_for(n, kPassThreads);
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_stage_hist, dim3((unsigned)blocks), dim3(kPassThreads),
+                       use_lds ? (size_t)nb * 4 : 0, s, z, n, e0, e1, inv, nb, hist, use_lds);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_interp_kick(const double *z, double *dE, int64_t n, const double *P,
+                                    double c0, double c1, double inv, int nb, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_interp_kick, dim3(blocks_for(n, 256)), dim3(256), 0, s, z, dE, n,
+                       P, c0, c1, inv, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_normals(uint64_t seed, int64_t turn, int64_t i0, int64_t n, double *out,
+                                hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_normals, dim3(blocks_for(n, 256)), dim3(256), 0, s, seed, turn, i0,
+                       n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate(uint64_t seed, double mu_z, double sz, double mu_E, double sE,
+                           double *z, double *dE, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256)), dim3(256), 0, s, seed, mu_z, sz, mu_E,
+                       sE, z, dE, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_turn_scalars(const BunchDev *bd, double E0, TurnScalars *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_turn_scalars, dim3(1), dim3(64), 0, s, bd, E0, out);
+    return hipGetLastError();
+}
+
+}  // namespace hm

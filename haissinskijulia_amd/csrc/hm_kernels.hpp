@@ -1,0 +1,59 @@
+// hm_kernels.hpp -- launch wrappers of the gfx950 kernels (hm_kernels.hip).
+// Every wrapper checks nothing itself: the engine validates shapes and
+// buffer sizes before it calls one.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hm_types.hpp"
+
+namespace hm {
+
+constexpr int kPassThreads = 256;      // turn-pass / moments / interp-sum block
+constexpr int kConvThreads = 256;      // convolution block
+constexpr int kFinalThreads = 256;     // per-bunch finalize block
+constexpr int kMaxFFT = 8192;          // largest LDS FFT (128 KiB of complex FP64)
+constexpr int kConvMaxPerThread = kMaxFFT / kConvThreads;
+
+// allow the dynamic-LDS kernels the device's LDS per block
+hipError_t configure_kernels(int max_dynamic_lds);
+
+// per-tile sums of z, z^2, dE, dE^2; mode 1 first applies the pending wake
+// kick and re-centre (the epilogue of the last turn)
+hipError_t launch_moments(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int mode,
+                          hipStream_t s);
+// per-bunch spreads from the tile sums; mode 0 -> out[0..1], mode 1 -> out[2..4]
+hipError_t launch_moments_final(const BunchDev *bds, const int32_t *bunch_list, int nb, int mode,
+                                hipStream_t s);
+hipError_t launch_init(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
+                            size_t lds_bytes, int threads, hipStream_t s);
+hipError_t launch_conv(const BunchDev *bds, const int32_t *conv_items, int nitems, int Fmax,
+                       hipStream_t s);
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
+                             hipStream_t s);
+hipError_t launch_finalize(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
+// FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
+hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,
+                           hipStream_t s);
+
+// stage kernels (unit parity entry points)
+hipError_t launch_stage_rf(double V, double sphis, double rf, double phis, const double *z,
+                           double *dE, int64_t n, hipStream_t s);
+hipError_t launch_stage_qe(double exc, const double *r, double *dE, int64_t n, hipStream_t s);
+hipError_t launch_stage_sr(double damp, double *dE, int64_t n, hipStream_t s);
+hipError_t launch_stage_drift(int update_eta, const TurnScalars *sc_dev, double mass,
+                              double alpha_c, double two_pi_h, double phis, double *z,
+                              const double *dE, int64_t n, hipStream_t s);
+hipError_t launch_stage_hist(const double *z, int64_t n, double e_first, double e_last,
+                             double inv_step, int nb, uint32_t *hist, int use_lds, hipStream_t s);
+hipError_t launch_stage_interp_kick(const double *z, double *dE, int64_t n, const double *P,
+                                    double c0, double c1, double inv, int nb, hipStream_t s);
+hipError_t launch_stage_normals(uint64_t seed, int64_t turn, int64_t i0, int64_t n, double *out,
+                                hipStream_t s);
+hipError_t launch_generate(uint64_t seed, double mu_z, double sz, double mu_E, double sE,
+                           double *z, double *dE, int64_t n, hipStream_t s);
+hipError_t launch_turn_scalars(const BunchDev *bd_dev, double E0, TurnScalars *out,
+                               hipStream_t s);
+
+}  // namespace hm

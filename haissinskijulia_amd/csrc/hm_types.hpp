@@ -1,0 +1,82 @@
+// hm_types.hpp -- layouts shared by the host engine and the gfx950 kernels.
+//
+// Device memory layout of one bunch (all FP64 unless noted):
+//   z[n], dE[n]                 particle state, SoA (data_structures.jl:26-29)
+//   hist[nbins]   (uint32)      line-density counts of the current turn
+//   P[nbins]                    wake potential at the bin centres (previous turn)
+//   lam[nbins]                  delta(c_i, 0.15) line-density kernel (wakefield.jl:219-221)
+//   What[nJ][F]   (complex)     FFT of the wake blocks, fixed for one evolve call
+//   part[ntiles][4]             per-tile sums  (sum z, sum z^2, sum dE'-dE_init, in-range count)
+//   ipart[ntiles]               per-tile sum of the interpolated potential (E0 update)
+//   moments[ntiles][4]          per-tile sums of z, z^2, dE, dE^2 (initial / final spreads)
+//   sc                          TurnScalars of the current turn
+//   out[8]                      sigma_E0, sigma_z0, sigma_E, sigma_z, E0, ...
+#pragma once
+#include <stdint.h>
+
+namespace hm {
+
+constexpr double kTwoPi = 6.283185307179586;   // Julia's 2π as Float64
+constexpr double kPi = 3.141592653589793;
+constexpr double kSpeedLight = 299792458.0;    // src/constants.jl:5
+
+enum BunchFlags : uint32_t {
+    F_WAKE = 1u,       // use_wakefield
+    F_ETA = 2u,        // update_eta
+    F_E0 = 4u,         // update_E0
+    F_SR = 8u,         // SR_damping
+    F_QE = 16u,        // use_excitation
+    F_INJ = 32u,       // injected normals instead of Philox
+    F_CHARGE = 64u,    // fixed wake current (charge extension / stage entry)
+    F_GHIST = 128u,    // histogram straight to global memory (too many bins for LDS)
+};
+
+// Scalars of one turn of one bunch (evolution.jl:300-304 plus the pending
+// state of the fused pass).  Written by k_init / k_finalize, read by the
+// per-particle kernels.
+struct TurnScalars {
+    double E0, gamma0, beta0, eta0, rf;
+    double denom;      // (beta0*beta0)*E0, drift denominator (evolution.jl:338)
+    double coeff_c;    // constant-eta phase coefficient (evolution.jl:157)
+    double damp;       // 1 - dU/dE (synchrotron_radiation.jl:44-45)
+    double exc;        // sqrt(1-(1-dU/dE)^2) * sigma_E0 (quantum_excitation.jl:58-59)
+    double m_prev;     // mean energy change still to subtract (evolution.jl:429-434)
+    double sigma_z, current, n_h;   // last bin phase (diagnostics)
+    int32_t pending_kick;   // 1: the previous turn's wake kick is still to be applied
+    int32_t pending_m;      // 1: m_prev still to be subtracted
+    int32_t drift_skip;     // apply_phase_advance! skip condition of this turn
+    int32_t drift_skipped_count;
+    int64_t turn;           // turns completed in this call
+    int64_t pad_;
+};
+
+struct BunchDev {
+    double *z, *dE;
+    int64_t n, npairs;
+    int64_t pairs_per_tile;
+    int32_t tile_begin, ntiles;
+    uint32_t flags;
+    int32_t nbins;
+    // physics constants of the call
+    double mass, voltage, alpha_c, phis, sin_phis, radius, freq_rf, two_pi_h;
+    double sigma_E0;   // initial energy spread (drives QE and the wake current)
+    double pf;         // particle factor of evolution.jl:377-383 (depends on n only)
+    double charge;     // fixed current when F_CHARGE
+    double E0_init;
+    uint64_t seed;
+    int64_t turn_offset;
+    const double *normals;   // device, [turns][n] when F_INJ
+    // histogram / interpolation grids (evolution.jl:270-274, wakefield.jl:38-42)
+    double e_first, e_last, inv_step;
+    double c_first, c_last, inv_s;
+    uint32_t *hist;
+    double *P;
+    const double *lam;
+    const double *What;   // interleaved complex [nJ][F][2]
+    const double *tw;     // interleaved complex [F/2][2], exp(-2 pi i k / F)
+    int32_t B, F, logF, nA, nJ, pad_;
+    double *part, *ipart, *moments, *out;
+    TurnScalars *sc;
+};
+
+}  // namespace hm

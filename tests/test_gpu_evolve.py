@@ -1,0 +1,163 @@
+"""Whole-loop parity of longitudinal_evolve on the GPU with the oracle.
+
+Level 1 of the parity plan: identical initial coordinates and identical
+injected QE normals; coordinates after 1 and 100 turns agree to 1e-12
+(max |difference| / rms of the oracle's coordinate).  The device Philox path
+is checked the same way against the oracle's restatement of that stream.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import COORD_RTOL, W, normwise, to_sim_params
+from oracle import oracle as O
+
+import haissinskijulia_amd as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = H.MI355X(0)
+    yield d
+    d.close()
+
+
+def _run_both(p, n, nbins, seed=1, normals=True, opts=None, oracle_kw=None, turn_offset=0):
+    z0, e0 = O.generate_gaussian(seed, 0.0, 4e-3, 0.0, 3e6, n)
+    nrm = None
+    if normals:
+        nrm = np.vstack([O.normals(seed + 100, t, 0, n) for t in range(p["n_turns"])]) \
+            if p["n_turns"] else np.zeros((0, n))
+    okw = dict(oracle_kw or {})
+    zr, er, rr = O.evolve(z0, e0, nbins, p, normals=nrm, seed=seed + 100, turn_offset=turn_offset, **okw)
+    assert rr.status == 0
+    part = H.Particles(z0.copy(), e0.copy())
+    bufs = H.create_simulation_buffers(n, max(nbins, 1))
+    o = opts or H.TrackingOptions()
+    o.seed = seed + 100
+    o.turn_offset = turn_offset
+    if normals:
+        o.normals = nrm
+    sE, sz, E0 = H.longitudinal_evolve(part, to_sim_params(p), bufs, dev, True, o)
+    return (zr, er, rr), (part.coordinates.z, part.coordinates.ΔE, sE, sz, E0)
+
+
+def _check(ref, got, rtol=COORD_RTOL):
+    (zr, er, rr), (zg, eg, sE, sz, E0) = ref, got
+    ez, ee = normwise(zg, zr), normwise(eg, er)
+    print(f"normwise z={ez:.2e} dE={ee:.2e}  E0 rel={abs(E0 - rr.E0) / abs(rr.E0):.2e}", flush=True)
+    assert ez <= rtol and ee <= rtol
+    assert E0 == pytest.approx(rr.E0, rel=1e-14)
+    assert sE == pytest.approx(rr.sigma_E, rel=1e-10)
+    assert sz == pytest.approx(rr.sigma_z, rel=1e-10)
+
+
+FLAGS = [(True, True, True, True, True), (True, False, True, True, True),
+         (True, True, False, True, True), (False, True, True, True, True),
+         (True, True, True, False, False), (False, False, False, False, False)]
+
+
+@pytest.mark.parametrize("turns", [1, 100])
+@pytest.mark.parametrize("flags", FLAGS)
+def test_reference_lattice_injected_normals(dev, turns, flags):
+    p = W.reference_benchmark_lattice(turns, flags)
+    print("case", turns, flags, flush=True)
+    _check(*_run_both(p, 4096, 256))
+
+
+@pytest.mark.parametrize("n,nbins", [(4097, 100), (1000, 1024), (3, 7), (2, 1), (50_000, 4096)])
+def test_ring_lattice_shapes(dev, n, nbins):
+    p = W.ring_3gev_lattice(n_turns=100)
+    print("case", n, nbins, flush=True)
+    _check(*_run_both(p, n, nbins))
+
+
+def test_device_philox_stream_matches_oracle(dev):
+    p = W.ring_3gev_lattice(n_turns=100)
+    _check(*_run_both(p, 8192, 512, normals=False, turn_offset=12345))
+
+
+def test_extensions_resonators_charge_half_width(dev):
+    p = W.ring_3gev_lattice(n_turns=100)
+    res = [W.RESONATOR_BB, W.RESONATOR_LOWQ]
+    opts = H.TrackingOptions(resonators=res, charge=1e-9, bin_half_width_sigmas=6.0, delta_sigma=-1.0)
+    ref, got = _run_both(p, 10_000, 1024, opts=opts,
+                         oracle_kw=dict(resonators=res, charge=1e-9, bin_half_width_sigmas=6.0,
+                                        delta_sigma=-1.0))
+    _check(ref, got)
+    # the wake changed the outcome (it is not a no-op)
+    p_nowake = dict(p, use_wakefield=False)
+    ref2, _ = _run_both(p_nowake, 10_000, 1024)
+    assert normwise(ref[0][1], ref2[0][1]) > 1e-6
+
+
+@pytest.mark.parametrize("fft_block", [64, 256])
+def test_fft_block_choice_does_not_change_result(dev, fft_block):
+    p = W.ring_3gev_lattice(n_turns=50)
+    opts = H.TrackingOptions(fft_block=fft_block)
+    _check(*_run_both(p, 20_000, 2048, opts=opts))
+
+
+def test_runs_are_bitwise_reproducible(dev):
+    p = H.SimulationParameters.from_dict(W.ring_3gev_lattice(n_turns=200))
+    z0, e0 = O.generate_gaussian(9, 0.0, 4e-3, 0.0, 3e6, 100_000)
+    outs = []
+    for _ in range(2):
+        part = H.Particles(z0.copy(), e0.copy())
+        r = H.longitudinal_evolve(part, p, H.create_simulation_buffers(z0.size, 1024), dev, True,
+                                  H.TrackingOptions(seed=5))
+        outs.append((part.coordinates.z.copy(), part.coordinates.ΔE.copy(), r))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]
+
+
+def test_zero_turns_returns_initial_spreads(dev):
+    p = W.ring_3gev_lattice(n_turns=0)
+    z0, e0 = O.generate_gaussian(2, 0.0, 4e-3, 0.0, 3e6, 1000)
+    part = H.Particles(z0.copy(), e0.copy())
+    sE, sz, E0 = H.longitudinal_evolve(part, to_sim_params(p), H.create_simulation_buffers(1000, 64), dev)
+    assert E0 == p["E0"]
+    assert sE == pytest.approx(O.global_std(e0), rel=1e-14)
+    assert sz == pytest.approx(O.global_std(z0), rel=1e-14)
+    np.testing.assert_array_equal(part.coordinates.z, z0)
+
+
+def test_error_codes_match_reference_errors(dev):
+    p = W.ring_3gev_lattice(n_turns=1)
+    z0, e0 = O.generate_gaussian(2, 0.0, 4e-3, 0.0, 3e6, 100)
+    part = H.Particles(z0.copy(), e0.copy())
+    with pytest.raises(H.HaissinskiError) as ei:
+        H.longitudinal_evolve(part, to_sim_params(dict(p, pipe_radius=0.0)),
+                              H.create_simulation_buffers(100, 16), dev)
+    assert ei.value.status == -4 and "pipe_radius must be positive" in str(ei.value)
+    with pytest.raises(H.HaissinskiError) as ei:
+        H.longitudinal_evolve(H.Particles(np.empty(0), np.empty(0)), to_sim_params(p),
+                              H.create_simulation_buffers(0, 16), dev)
+    assert ei.value.status == -2
+    r = H.longitudinal_evolve(H.Particles(np.empty(0), np.empty(0)),
+                              to_sim_params(dict(p, use_wakefield=False)),
+                              H.create_simulation_buffers(0, 16), dev)
+    assert math.isnan(r[2])
+
+
+def test_large_bunch_matches_oracle_for_three_turns(dev):
+    # full-size parity (1e7 particles, 8192 bins), oracle on 16 threads
+    p = W.ring_3gev_lattice(n_turns=3)
+    n = 10_000_000
+    z0, e0 = O.generate_gaussian(7, 0.0, 4e-3, 0.0, 3e6, n)
+    res = [W.RESONATOR_BB]
+    okw = dict(resonators=res, charge=5e-9, bin_half_width_sigmas=8.0, delta_sigma=-1.0)
+    zr, er, rr = O.evolve(z0, e0, 8192, p, seed=7, nranks=16, **okw)
+    part = H.Particles(z0, e0.copy())
+    opts = H.TrackingOptions(seed=7, resonators=res, charge=5e-9, bin_half_width_sigmas=8.0,
+                             delta_sigma=-1.0)
+    sE, sz, E0 = H.longitudinal_evolve(part, to_sim_params(p), H.create_simulation_buffers(n, 8192),
+                                       dev, True, opts)
+    ez, ee = normwise(part.coordinates.z, zr), normwise(part.coordinates.ΔE, er)
+    print(f"1e7: normwise z={ez:.2e} dE={ee:.2e}", flush=True)
+    assert ez <= COORD_RTOL and ee <= COORD_RTOL
+    assert E0 == pytest.approx(rr.E0, rel=1e-14)
