@@ -1,0 +1,58 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py).
+
+CPU: the oracle still reproduces every fixture bit for bit.
+GPU: the HIP path matches every fixture within the parity tolerance.
+"""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import COORD_RTOL, normwise
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "*.npz")))
+
+
+def _load(path):
+    d = np.load(path, allow_pickle=False)
+    return d, json.loads(str(d["params"])), json.loads(str(d["case"]))
+
+
+def _normals(case):
+    from oracle import oracle as O
+    return np.vstack([O.normals(1000 + case["seed"], t, 0, case["n"]) for t in range(case["turns"])])
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(f)[:-4] for f in FIXTURES])
+def test_oracle_reproduces_fixture(path):
+    from oracle import oracle as O
+    d, p, case = _load(path)
+    z, e, r = O.evolve(d["z0"], d["dE0"], case["nbins"], p, normals=_normals(case), **case.get("ext", {}))
+    np.testing.assert_array_equal(z, d["z"])
+    np.testing.assert_array_equal(e, d["dE"])
+    np.testing.assert_array_equal([r.sigma_E, r.sigma_z, r.E0], d["scalars"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(f)[:-4] for f in FIXTURES])
+def test_gpu_matches_fixture(path):
+    import haissinskijulia_amd as H
+    d, p, case = _load(path)
+    print("fixture", os.path.basename(path), flush=True)
+    part = H.Particles(d["z0"].copy(), d["dE0"].copy())
+    ext = case.get("ext", {})
+    opts = H.TrackingOptions(normals=_normals(case), resonators=[tuple(r) for r in ext.get("resonators", [])],
+                             charge=ext.get("charge", 0.0),
+                             bin_half_width_sigmas=ext.get("bin_half_width_sigmas", 0.0),
+                             delta_sigma=ext.get("delta_sigma", 0.0))
+    sE, sz, E0 = H.longitudinal_evolve(part, H.SimulationParameters.from_dict(p),
+                                       H.create_simulation_buffers(case["n"], max(case["nbins"], 1)),
+                                       H.MI355X.default(), True, opts)
+    assert normwise(part.coordinates.z, d["z"]) <= COORD_RTOL
+    assert normwise(part.coordinates.ΔE, d["dE"]) <= COORD_RTOL
+    assert E0 == pytest.approx(d["scalars"][2], rel=1e-14)
+    assert sE == pytest.approx(d["scalars"][0], rel=1e-10)
+    assert sz == pytest.approx(d["scalars"][1], rel=1e-10)

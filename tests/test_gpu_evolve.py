@@ -20,9 +20,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def dev():
-    d = H.MI355X(0)
-    yield d
-    d.close()
+    return H.MI355X.default()
 
 
 def _run_both(p, n, nbins, seed=1, normals=True, opts=None, oracle_kw=None, turn_offset=0):
@@ -41,7 +39,7 @@ def _run_both(p, n, nbins, seed=1, normals=True, opts=None, oracle_kw=None, turn
     o.turn_offset = turn_offset
     if normals:
         o.normals = nrm
-    sE, sz, E0 = H.longitudinal_evolve(part, to_sim_params(p), bufs, dev, True, o)
+    sE, sz, E0 = H.longitudinal_evolve(part, to_sim_params(p), bufs, H.MI355X.default(), True, o)
     return (zr, er, rr), (part.coordinates.z, part.coordinates.ΔE, sE, sz, E0)
 
 
