@@ -107,12 +107,15 @@ def main() -> int:
     log(f"timed {args.steps} turns: {wall:.3f} s -> {value:.3e} particle-turns/s "
         f"(device {r.ms_total:.1f} ms), sigma_E={r.sigma_E:.4e} sigma_z={r.sigma_z:.4e} E0={r.E0:.6e}")
 
-    rt = bunch.evolve(params(args.timing_turns), nbins, tracking_options(wl, wl["seed"] + 2, True))
+    rt = bunch.evolve(params(max(1, args.timing_turns)), nbins,
+                      tracking_options(wl, wl["seed"] + 2, True))
     pass_ms = rt.ms_pass_avg
     achieved = BYTES_PER_PARTICLE_TURN * n / (pass_ms * 1e-3) / 1e9
     log(f"turn-pass kernel: {pass_ms:.4f} ms avg over {rt.pass_launches} launches -> {achieved:.0f} GB/s; "
         f"convolution {rt.ms_conv_avg:.4f} ms")
     bunch.close()
+    copy_ms, copy_gbs = H.api.stream_copy(int(16 * n), 10, dev)
+    log(f"plain copy of {16 * n / 1e9:.1f} GB: {copy_ms:.3f} ms -> {copy_gbs:.0f} GB/s")
 
     out = {
         "metric": METRIC,
@@ -138,6 +141,7 @@ def main() -> int:
                      "kernel": "k_turn_pass", "kernel_ms": pass_ms,
                      "algorithmic_bytes_per_launch": BYTES_PER_PARTICLE_TURN * n},
         "whole_loop_frac_of_hbm_peak": value * BYTES_PER_PARTICLE_TURN / 1e9 / HBM_PEAK_GBS,
+        "stream_copy_GBps": copy_gbs,
         "conv_ms": rt.ms_conv_avg,
     }
     if not args.no_cpu_baseline:

@@ -13,7 +13,9 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhaissinski_mi355x.so")
+# HAISSINSKI_MI355X_LIB selects another build of the same library (e.g. a
+# variant compiled for a measurement); the default is the in-tree build.
+LIB_PATH = os.environ.get("HAISSINSKI_MI355X_LIB") or os.path.join(HERE, "libhaissinski_mi355x.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "haissinski_mi355x.h")
 
 _dp = C.POINTER(C.c_double)
@@ -126,6 +128,7 @@ def load() -> C.CDLL:
             "hm_global_std": (st, [vp, _dp, C.c_int64, _dp]),
             "hm_normals": (st, [vp, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, _dp]),
             "hm_bin_grid": (st, [C.c_double, C.c_int64, _dp, _dp, _dp]),
+            "hm_stream_copy": (st, [vp, C.c_int64, C.c_int32, _dp, _dp]),
             "hm_server_create": (st, [vp, C.c_int64, C.c_int32, C.POINTER(vp)]),
             "hm_server_destroy": (st, [vp]),
             "hm_server_submit": (st, [vp, C.POINTER(HmJob), C.POINTER(C.c_int64)]),

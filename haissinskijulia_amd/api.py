@@ -492,6 +492,14 @@ def compute_global_std(x: np.ndarray, comm=None) -> float:
     return out.value
 
 
+def stream_copy(nbytes: int, iters: int = 20, comm=None) -> tuple[float, float]:
+    """(ms per copy, GB/s read+write) of a plain device copy of nbytes."""
+    ms = C.c_double()
+    gbs = C.c_double()
+    check(_lib.load().hm_stream_copy(_dev(comm).ctx, int(nbytes), int(iters), C.byref(ms), C.byref(gbs)))
+    return ms.value, gbs.value
+
+
 def bin_grid(z_width: float, nbins: int):
     """range(-zw, zw, length=nbins+1) and its centres (evolution.jl:273-274),
     correctly rounded.  Host only."""

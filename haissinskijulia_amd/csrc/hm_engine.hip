@@ -153,10 +153,14 @@ int max_fft_block(const hm_ctx *ctx) {
 
 // Convolution block size: the wake support fits one block when it can, so
 // every output block needs one forward and one inverse FFT.
+// A block of kAutoFFTBlock keeps the transforms small enough that the
+// blocked convolution spreads over many workgroups; the choice depends on the
+// bunch's own table only, so results do not depend on the batch.
 int auto_fft_block(int64_t nw, int64_t nbins, int max_block) {
     (void)nbins;
+    const int cap = std::min(max_block, kAutoFFTBlock);
     int Bk = 64;
-    while (Bk < nw && Bk < max_block) Bk <<= 1;
+    while (Bk < nw && Bk < cap) Bk <<= 1;
     return Bk;
 }
 
@@ -197,8 +201,8 @@ hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t n
         }
     }
     if (o && o->n_resonators > 0 && !o->resonators) return fail(HM_ERR_ARG, "resonators is NULL");
-    if (o && o->block_size && (o->block_size % 64 || o->block_size > 1024 || o->block_size < 64))
-        return fail(HM_ERR_ARG, "block_size must be a multiple of 64 in [64, 1024]");
+    if (o && o->block_size && o->block_size != kPassThreads)
+        return fail(HM_ERR_ARG, "block_size must be 0 or %d", kPassThreads);
     return HM_OK;
 }
 
@@ -217,7 +221,8 @@ static int64_t pairs_per_tile_for(int64_t npairs, int threads) {
 
 Batch::~Batch() {
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
-    d_bd_.release(); d_tiles_.release(); d_conv_.release(); d_list_.release();
+    d_bd_.release(); d_tiles_.release(); d_list_.release();
+    for (auto &c : d_conv_) c.release();
     d_tmp_tiles_.release(); d_tmp_list_.release();
 }
 
@@ -241,38 +246,58 @@ hm_status Batch::upload_descriptors() {
 
 // launch tables of the active bunches
 hm_status Batch::build_tables() {
-    std::vector<int32_t> tiles, conv, list;
+    std::vector<int32_t> tiles, conv[3], list;
     pass_lds_ = 0;
     Fmax_ = 2;
-    any_wake_ = any_wake_e0_ = false;
+    any_wake_ = any_wake_e0_ = any_plds_ = false;
+    std_only_ = true;
+    interp_lds_ = 0;
     for (size_t i = 0; i < rb_.size(); ++i) {
         RunBunch &q = rb_[i];
         if (!used_[i] || !q.active) continue;
         q.d.tile_begin = (int32_t)tiles.size();
         for (int t = 0; t < q.d.ntiles; ++t) tiles.push_back((int32_t)i);
         list.push_back((int32_t)i);
+        if ((q.d.flags & kPhysMask) != kStdFlags) std_only_ = false;
         if (q.d.flags & F_WAKE) {
             any_wake_ = true;
-            if (q.d.flags & F_E0) any_wake_e0_ = true;
+            if (q.d.flags & F_E0) {
+                if (q.d.flags & F_PLDS) {
+                    any_plds_ = true;
+                    interp_lds_ = std::max(interp_lds_, (size_t)q.d.nbins * 8);
+                } else {
+                    any_wake_e0_ = true;
+                }
+            }
             for (int o = 0; o < q.d.nA; ++o) {
-                conv.push_back((int32_t)i);
-                conv.push_back(o);
+                if (q.d.nJ <= 1) {
+                    conv[CONV_FUSED].push_back((int32_t)i);
+                    conv[CONV_FUSED].push_back(o);
+                } else {
+                    conv[CONV_FWD].push_back((int32_t)i);
+                    conv[CONV_FWD].push_back(o);
+                    conv[CONV_INV].push_back((int32_t)i);
+                    conv[CONV_INV].push_back(o);
+                }
             }
             Fmax_ = std::max(Fmax_, q.d.F);
             if (!(q.d.flags & F_GHIST)) pass_lds_ = std::max(pass_lds_, (size_t)q.d.nbins * 4);
         }
     }
     ntiles_ = (int)tiles.size();
-    nconv_ = (int)conv.size() / 2;
     nlist_ = (int)list.size();
     hipStream_t s = ctx_->stream;
     HM_TRY(d_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
-    HM_TRY(d_conv_.ensure(std::max<size_t>(conv.size(), 2)));
     HM_TRY(d_list_.ensure(std::max<size_t>(list.size(), 1)));
+    for (int k = 0; k < 3; ++k) {
+        nconv_[k] = (int)conv[k].size() / 2;
+        HM_TRY(d_conv_[k].ensure(std::max<size_t>(conv[k].size(), 2)));
+        if (!conv[k].empty())
+            HIP_TRY(hipMemcpyAsync(d_conv_[k].p, conv[k].data(), conv[k].size() * 4,
+                                   hipMemcpyHostToDevice, s));
+    }
     if (!tiles.empty())
         HIP_TRY(hipMemcpyAsync(d_tiles_.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, s));
-    if (!conv.empty())
-        HIP_TRY(hipMemcpyAsync(d_conv_.p, conv.data(), conv.size() * 4, hipMemcpyHostToDevice, s));
     if (!list.empty())
         HIP_TRY(hipMemcpyAsync(d_list_.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
     HM_TRY(upload_descriptors());  // synchronises: the vectors above are still alive
@@ -344,7 +369,7 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         d.dE = b->dE;
         d.n = b->n;
         d.npairs = (b->n + 1) / 2;
-        d.pairs_per_tile = pairs_per_tile_for(d.npairs, threads_);
+        d.pairs_per_tile = pairs_per_tile_for(d.npairs, kPassThreads);
         d.ntiles = (int32_t)((d.npairs + d.pairs_per_tile - 1) / d.pairs_per_tile);
         uint32_t fl = 0;
         if (p->use_wakefield) fl |= F_WAKE;
@@ -356,6 +381,7 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         if (q.o.charge > 0) fl |= F_CHARGE;
         d.flags = fl;
         d.mass = p->mass;
+        d.inv_mass = 1.0 / p->mass;
         d.voltage = p->voltage;
         d.alpha_c = p->alpha_c;
         d.phis = p->phi_s;
@@ -464,15 +490,23 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         d.nA = (int32_t)((nb + Bk - 1) / Bk);
         d.nJ = (int32_t)((nw + Bk - 1) / Bk);
         if ((size_t)nb * 4 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
+        if ((size_t)nb * 8 <= (size_t)ctx_->lds_max - 1024) d.flags |= F_PLDS;
         Work &w = q.b->w;
         HM_TRY(w.hist.ensure((size_t)nb));
         HM_TRY(w.P.ensure((size_t)nb));
+        HM_TRY(w.P2.ensure((size_t)nb * 2));
         HM_TRY(w.lam.ensure((size_t)nb));
         HM_TRY(w.Wtmp.ensure((size_t)nb));
         HM_TRY(w.What.ensure((size_t)std::max(d.nJ, 1) * (size_t)d.F * 2));
+        if (d.nJ > 1) {
+            HM_TRY(w.X.ensure((size_t)d.nA * (size_t)d.F * 2));
+            d.X = w.X.p;
+        }
         HIP_TRY(hipMemcpyAsync(w.lam.p, lam.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(w.Wtmp.p, W.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemsetAsync(w.P.p, 0, (size_t)nb * 8, s));
+        HIP_TRY(hipMemsetAsync(w.P2.p, 0, (size_t)nb * 16, s));
+        d.P2 = w.P2.p;
         keep.push_back(std::move(W));
         keep.push_back(std::move(lam));
         d.hist = w.hist.p;
@@ -520,11 +554,18 @@ hm_status Batch::one_turn() {
         ev_used_ += 3;
         HIP_TRY(hipEventRecord(e0, s));
     }
-    HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, ntiles_, pass_lds_, threads_, s));
+    HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, ntiles_, pass_lds_, std_only_ ? 1 : 0, s));
     if (timing_) HIP_TRY(hipEventRecord(e1, s));
-    if (any_wake_) HIP_TRY(launch_conv(d_bd_.p, d_conv_.p, nconv_, Fmax_, s));
+    if (any_wake_) {
+        HIP_TRY(launch_conv(CONV_FUSED, d_bd_.p, d_conv_[CONV_FUSED].p, nconv_[CONV_FUSED], Fmax_, s));
+        HIP_TRY(launch_conv(CONV_FWD, d_bd_.p, d_conv_[CONV_FWD].p, nconv_[CONV_FWD], Fmax_, s));
+        HIP_TRY(launch_conv(CONV_INV, d_bd_.p, d_conv_[CONV_INV].p, nconv_[CONV_INV], Fmax_, s));
+    }
     if (timing_) HIP_TRY(hipEventRecord(e2, s));
     if (any_wake_e0_) HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, ntiles_, s));
+    if (any_plds_)
+        HIP_TRY(launch_interp_sum_lds(d_bd_.p, d_tiles_.p, ntiles_, std::min(ntiles_, ctx_->cus),
+                                      interp_lds_, s));
     HIP_TRY(launch_finalize(d_bd_.p, d_list_.p, nlist_, s));
     return HM_OK;
 }
@@ -616,7 +657,6 @@ static hm_status run_batch(hm_ctx *ctx, int32_t nb, hm_bunch *const *bunches, co
                            const hm_params *params, const hm_options *opts, hm_result *results) {
     if (!ctx || nb <= 0 || !bunches || !params || !results) return fail(HM_ERR_ARG, "bad batch arguments");
     HM_TRY(set_device(ctx));
-    int threads = kPassThreads;
     bool timing = false;
     std::vector<RunBunch> specs;
     std::vector<int32_t> orig;
@@ -630,7 +670,6 @@ static hm_status run_batch(hm_ctx *ctx, int32_t nb, hm_bunch *const *bunches, co
         q.nbins = nbins ? nbins[i] : 0;
         HM_TRY(validate(&q.p, &q.o, q.b->n, q.nbins));
         if (q.o.timing) timing = true;
-        if (q.o.block_size) threads = q.o.block_size;
         if (q.b->n == 0) {   // no wake (validated): E0 becomes 0/0 after an update (evolution.jl:426)
             hm_result &r = results[i];
             std::memset(&r, 0, sizeof(r));
@@ -642,7 +681,7 @@ static hm_status run_batch(hm_ctx *ctx, int32_t nb, hm_bunch *const *bunches, co
         orig.push_back(i);
     }
     if (specs.empty()) return HM_OK;
-    Batch B(ctx, threads);
+    Batch B(ctx);
     B.set_timing(timing);
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));

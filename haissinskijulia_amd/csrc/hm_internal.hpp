@@ -60,17 +60,17 @@ struct DevBuf {
 // per-bunch work buffers of an evolve call (kept between calls)
 struct Work {
     DevBuf<uint32_t> hist;
-    DevBuf<double> P, lam, What, Wtmp, part, ipart, moments, normals, out;
+    DevBuf<double> P, P2, lam, What, X, Wtmp, part, ipart, moments, normals, out;
     DevBuf<TurnScalars> sc;
     void release() {
-        hist.release(); P.release(); lam.release(); What.release(); Wtmp.release();
-        part.release(); ipart.release(); moments.release(); normals.release(); out.release();
-        sc.release();
+        hist.release(); P.release(); P2.release(); lam.release(); What.release(); X.release();
+        Wtmp.release(); part.release(); ipart.release(); moments.release(); normals.release();
+        out.release(); sc.release();
     }
     size_t bytes() const {
-        return hist.bytes() + P.bytes() + lam.bytes() + What.bytes() + Wtmp.bytes() +
-               part.bytes() + ipart.bytes() + moments.bytes() + normals.bytes() + out.bytes() +
-               sc.bytes();
+        return hist.bytes() + P.bytes() + P2.bytes() + lam.bytes() + What.bytes() + X.bytes() +
+               Wtmp.bytes() + part.bytes() + ipart.bytes() + moments.bytes() + normals.bytes() +
+               out.bytes() + sc.bytes();
     }
 };
 
@@ -130,7 +130,7 @@ struct RunBunch {
 // finalize launch.  Bunches may join (add) and leave (remove) between turns.
 class Batch {
   public:
-    explicit Batch(hm_ctx *ctx, int threads = kPassThreads) : ctx_(ctx), threads_(threads) {}
+    explicit Batch(hm_ctx *ctx) : ctx_(ctx) {}
     ~Batch();
     // set up new bunches (initial spreads, grids, tables, FFT plan); their
     // slot indices are appended to idx
@@ -159,15 +159,19 @@ class Batch {
     hm_status flush_timing();
 
     hm_ctx *ctx_;
-    int threads_;
     std::vector<RunBunch> rb_;
     std::vector<bool> used_;
     DevBuf<BunchDev> d_bd_;
-    DevBuf<int32_t> d_tiles_, d_conv_, d_list_, d_tmp_tiles_, d_tmp_list_;
-    int ntiles_ = 0, nconv_ = 0, nlist_ = 0;
+    DevBuf<int32_t> d_tiles_, d_list_, d_tmp_tiles_, d_tmp_list_;
+    DevBuf<int32_t> d_conv_[3];   // fused, forward, inverse convolution items
+    int ntiles_ = 0, nlist_ = 0;
+    int nconv_[3] = {0, 0, 0};
     size_t pass_lds_ = 0;
     int Fmax_ = 2;
-    bool any_wake_ = false, any_wake_e0_ = false;
+    bool any_wake_ = false, any_wake_e0_ = false;   // wake-E0 bunches with P in global memory
+    bool any_plds_ = false;                         // wake-E0 bunches with P staged in LDS
+    bool std_only_ = true;                          // all active bunches have kStdFlags
+    size_t interp_lds_ = 0;
     bool dirty_ = true;
     bool timing_ = false;
     std::vector<hipEvent_t> ev_pool_;

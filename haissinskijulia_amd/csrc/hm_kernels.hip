@@ -8,22 +8,35 @@
 //                 drift (evolution.jl:330-347); it bins the new z into an
 //                 LDS-privatised histogram (utils.jl:351-401) and writes
 //                 deterministic per-tile partial sums.
-//   k_conv        the bin phase: sigma_z, the wake current (evolution.jl:373-386)
+//   k_conv_*      the bin phase: sigma_z, the wake current (evolution.jl:373-386)
 //                 and the line-density convolution with the wake
-//                 (wakefield.jl:186-319) by a blocked LDS FFT.
+//                 (wakefield.jl:186-319) by blocked LDS FFTs.
 //   k_interp_sum  (update_E0 with the wake) sum of the interpolated potential
 //                 over the post-drift z, which the E0 update of
 //                 evolution.jl:404-435 needs before the next turn.
 //   k_finalize    E0 update, scalars of the next turn, histogram reset.
 //
-// All floating-point work is FP64 with no contraction (-ffp-contract=off);
-// sums are reduced in a fixed order, so a run is bitwise reproducible.
+// All floating-point work is FP64.  Multiplies and adds are never contracted
+// (-ffp-contract=off) except inside the elementary functions of hm_math.hpp;
+// divides by per-turn constants are multiplies by their reciprocals.  Sums
+// are reduced in a fixed order, so a run is bitwise reproducible.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include "hm_kernels.hpp"
 
+#define HM_QUAL __device__ __forceinline__
+#include "hm_math.hpp"
+
 namespace hm {
+
+struct DevOps {
+    static __device__ __forceinline__ double rcp(double a) { return __builtin_amdgcn_rcp(a); }
+    static __device__ __forceinline__ double frexp_mant(double u) {
+        return __builtin_amdgcn_frexp_mant(u);
+    }
+    static __device__ __forceinline__ int frexp_exp(double u) { return __builtin_amdgcn_frexp_exp(u); }
+};
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -54,7 +67,9 @@ __device__ inline void turn_scalars(const BunchDev &B, double E0, TurnScalars &s
     s.beta0 = sqrt(jl_max(0.0, 1.0 - ig2));
     s.eta0 = B.alpha_c - ig2;
     s.rf = (s.beta0 <= 0.0) ? INFINITY : (B.freq_rf * kTwoPi) / (s.beta0 * kSpeedLight);
+    s.inv_rf = 1.0 / s.rf;
     s.denom = (s.beta0 * s.beta0) * E0;
+    s.kcoef = B.two_pi_h / s.denom;
     s.coeff_c = (B.two_pi_h * s.eta0) / s.denom;
     const double x = E0 / 1e9;
     const double d = ((4.0 * 8.85e-5) * ((x * x) * x)) / B.radius;
@@ -64,17 +79,39 @@ __device__ inline void turn_scalars(const BunchDev &B, double E0, TurnScalars &s
     s.drift_skip = (!(B.flags & F_ETA)) && (s.beta0 <= 0.0 || E0 == 0.0 || s.rf == 0.0);
 }
 
-// Philox4x32-10 (Salmon et al., SC'11)
-__device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                     uint32_t k0, uint32_t k1, uint32_t x[4]) {
+// sin for the RF kick; arguments far from the bucket (|x| >= 2^19) go to the
+// device library, out of line so the common path keeps its registers
+__device__ __noinline__ double sin_far(double x) { return sin(x); }
+
+__device__ __forceinline__ double dsin(double x) {
+    if (__builtin_expect(!(__builtin_fabs(x) < fm::kReduceMax), 0)) return sin_far(x);
+    return fm::sin_reduced(x);
+}
+
+// Physics flags of a pass: FX != 0 fixes them at compile time (the common
+// all-on configuration), FX == 0 reads them from the bunch.
+template <uint32_t FX>
+struct Flags {
+    uint32_t rt;
+    __device__ __forceinline__ bool has(uint32_t f) const {
+        return FX ? ((FX & f) != 0) : ((rt & f) != 0);
+    }
+};
+
+// Philox4x32-10 (Salmon et al., SC'11); the 32x32 -> 64 products give the
+// high and low words in one multiply
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t x[4]) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-        c0 = hi1 ^ c1 ^ k0;
-        c1 = lo1;
-        c2 = hi0 ^ c3 ^ k1;
-        c3 = lo0;
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -82,43 +119,50 @@ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint
 }
 
 // Box-Muller (1958) on one Philox block: u1 in (0,1], u2 in [0,1), 53 bits each
-__device__ inline void box_muller(const uint32_t x[4], double &n0, double &n1) {
+__device__ __forceinline__ void box_muller(const uint32_t x[4], double &n0, double &n1) {
     const double u1 = ((double)(x[0] >> 5) * 67108864.0 + (double)(x[1] >> 6) + 1.0) * 0x1p-53;
     const double u2 = ((double)(x[2] >> 5) * 67108864.0 + (double)(x[3] >> 6)) * 0x1p-53;
-    const double r = sqrt(-2.0 * log(u1));
+    const double r = sqrt(-2.0 * fm::log_pos<DevOps>(u1));
     double s, c;
-    sincos(kTwoPi * u2, &s, &c);
+    fm::sincos_reduced(kTwoPi * u2, s, c);
     n0 = r * c;
     n1 = r * s;
 }
 
 // QE normals of particle pair p at absolute turn t
-__device__ inline void qe_pair_normals(uint64_t seed, uint64_t p, uint32_t t, double &n0,
-                                       double &n1) {
+__device__ __forceinline__ void qe_pair_normals(uint64_t seed, uint64_t p, uint32_t t, double &n0,
+                                                double &n1) {
     uint32_t x[4];
-    philox4x32_10((uint32_t)p, (uint32_t)(p >> 32), t, 0u, (uint32_t)seed,
-                  (uint32_t)(seed >> 32), x);
+    philox4x32_10((uint32_t)p, (uint32_t)(p >> 32), t, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), x);
     box_muller(x, n0, n1);
 }
 
-// zero_alloc_interpolation! for one particle (wakefield.jl:32-36,38-42,64-78,94-102)
-__device__ inline double interp_pot(const double *__restrict__ P, double z, double c0, double c1,
-                                    double inv, int nb) {
-    if (nb <= 1) return nb == 1 ? P[0] : 0.0;
+// zero_alloc_interpolation! index and weight (wakefield.jl:38-42,64-78) for nb >= 2
+__device__ __forceinline__ void interp_index(double z, double c0, double c1, double inv, int nb,
+                                             int &k, double &w) {
     const double cl = (z > c1) ? c1 : ((z < c0) ? c0 : z);
     const double u = (cl - c0) * inv;
     const double fu = floor(u);
-    int k = (fu == fu) ? (int)fu : 0;
-    k = k < 0 ? 0 : (k > nb - 2 ? nb - 2 : k);
-    const double w = u - fu;
+    int kk = (fu == fu) ? (int)fu : 0;
+    k = kk < 0 ? 0 : (kk > nb - 2 ? nb - 2 : kk);
+    w = u - fu;
+}
+
+// the interpolated value (wakefield.jl:94-102)
+__device__ __forceinline__ double interp_pot(const double *__restrict__ P, double z, double c0,
+                                             double c1, double inv, int nb) {
+    if (nb <= 1) return nb == 1 ? P[0] : 0.0;
+    int k;
+    double w;
+    interp_index(z, c0, c1, inv, nb, k, w);
     return P[k] * (1.0 - w) + P[k + 1] * w;
 }
 
 // histogram bin of calculate_local_histogram! (utils.jl:387-397); -1 = not counted
-__device__ inline int hist_bin(double z, double e0, double e1, double inv, int nb) {
+__device__ __forceinline__ int hist_bin(double z, double e0, double e1, double inv, int nb) {
     if (!((z >= e0) & (z <= e1))) return -1;
     const double u = (z - e0) * inv;
-    int b = (int)floor(u);
+    const int b = (int)floor(u);
     return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
 }
 
@@ -147,11 +191,13 @@ __device__ inline void block_sum(double (&v)[NV], double *s_red) {
     }
 }
 
-__device__ inline double2 cmul(double2 a, double2 b) {
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-__device__ inline int bitrev(int m, int logF) { return (int)(__brev((unsigned)m) >> (32 - logF)); }
+__device__ __forceinline__ int bitrev(int m, int logF) {
+    return (int)(__brev((unsigned)m) >> (32 - logF));
+}
 
 // In-place radix-2 decimation-in-time FFT of X[F] in LDS, bit-reversed input,
 // natural-order output, twiddles tw[k] = exp(-2 pi i k / F).
@@ -196,12 +242,10 @@ __device__ void fft_dif_inverse(double2 *X, int F, int logF, const double2 *__re
 // ---------------------------------------------------------------------------
 
 struct PassCtx {
-    // current turn
-    double rf, phis, sin_phis, voltage, exc, damp, mass, gamma0, alpha_c, two_pi_h, denom,
+    double rf, inv_rf, phis, sin_phis, voltage, exc, damp, inv_mass, gamma0, alpha_c, kcoef,
         coeff_c, m_prev;
     int pending_kick, pending_m, drift_skip;
     uint32_t flags;
-    // grids
     const double *P;
     double c0, c1, inv_s, e0, e1, inv_step;
     int nb;
@@ -210,16 +254,16 @@ struct PassCtx {
 __device__ inline void load_ctx(const BunchDev &B, PassCtx &c) {
     const TurnScalars &s = *B.sc;
     c.rf = s.rf;
+    c.inv_rf = s.inv_rf;
     c.phis = B.phis;
     c.sin_phis = B.sin_phis;
     c.voltage = B.voltage;
     c.exc = s.exc;
     c.damp = s.damp;
-    c.mass = B.mass;
+    c.inv_mass = B.inv_mass;
     c.gamma0 = s.gamma0;
     c.alpha_c = B.alpha_c;
-    c.two_pi_h = B.two_pi_h;
-    c.denom = s.denom;
+    c.kcoef = s.kcoef;
     c.coeff_c = s.coeff_c;
     c.m_prev = s.m_prev;
     c.pending_kick = s.pending_kick;
@@ -238,46 +282,66 @@ __device__ inline void load_ctx(const BunchDev &B, PassCtx &c) {
 
 // The previous turn's wake kick (wakefield.jl:342-344) and E0 re-centre
 // (evolution.jl:434), applied at the start of this pass.
-__device__ inline double apply_pending(const PassCtx &c, double z, double e) {
+__device__ __forceinline__ double apply_pending(const PassCtx &c, double z, double e) {
     if (c.pending_kick) e = e - interp_pot(c.P, z, c.c0, c.c1, c.inv_s, c.nb);
     if (c.pending_m) e = e - c.m_prev;
     return e;
 }
 
-// RF, QE, SR and drift of one particle; returns dE' - dE_init via de_turn.
-__device__ inline void advance(const PassCtx &c, double &z, double &e, double r, double &de_turn) {
-    const double e_init = e;
-    const double phi0 = ((-z) * c.rf) + c.phis;                    // rf_kick.jl:85
-    e = e + c.voltage * (sin(phi0) - c.sin_phis);                  // rf_kick.jl:86
-    if (c.flags & F_QE) e = e + c.exc * r;                         // quantum_excitation.jl:70
-    if (c.flags & F_SR) e = e * c.damp;                            // synchrotron_radiation.jl:55
-    if (c.flags & F_ETA) {                                         // evolution.jl:332-343
-        const double dg = e / c.mass;
-        const double gp = c.gamma0 + dg;
-        const double eta = c.alpha_c - 1.0 / (gp * gp);
-        const double coeff = (c.two_pi_h * eta) / c.denom;
+// phase advance (evolution.jl:331-344 with update_eta, :159-165 without)
+template <uint32_t FX>
+__device__ __forceinline__ double drift(const PassCtx &c, Flags<FX> fl, double z, double e) {
+    if (fl.has(F_ETA)) {
+        const double gp = c.gamma0 + e * c.inv_mass;
+        // 1/gp^2 ~ 3e-8 enters eta ~ 1e-3 as a correction: one Newton step
+        const double eta = c.alpha_c - fm::rcp1<DevOps>(gp * gp);
+        const double coeff = eta * c.kcoef;
         double phi = -((z * c.rf) - c.phis);
         phi = phi + coeff * e;
-        z = ((-phi) + c.phis) / c.rf;
-    } else if (!c.drift_skip) {                                    // evolution.jl:159-165
+        return ((-phi) + c.phis) * c.inv_rf;
+    } else if (!c.drift_skip) {
         double phi = -((z * c.rf) - c.phis);
         phi = phi + c.coeff_c * e;
-        z = ((-phi) + c.phis) / c.rf;
+        return ((-phi) + c.phis) * c.inv_rf;
     }
+    return z;
+}
+
+// RF, QE, SR and drift of one particle; returns dE' - dE_init via de_turn.
+template <uint32_t FX>
+__device__ __forceinline__ void advance(const PassCtx &c, Flags<FX> fl, double &z, double &e,
+                                        double r, double &de_turn) {
+    const double e_init = e;
+    const double phi0 = ((-z) * c.rf) + c.phis;                    // rf_kick.jl:85
+    e = e + c.voltage * (dsin(phi0) - c.sin_phis);                 // rf_kick.jl:86
+    if (fl.has(F_QE)) e = e + c.exc * r;                           // quantum_excitation.jl:70
+    if (fl.has(F_SR)) e = e * c.damp;                              // synchrotron_radiation.jl:55
+    z = drift(c, fl, z, e);
     de_turn = e - e_init;
 }
 
-__global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__restrict__ bds,
-                                                            const int32_t *__restrict__ tile_bunch) {
-    extern __shared__ uint32_t s_hist[];
-    __shared__ double s_red[4 * 16];
-    const int tile = blockIdx.x;
-    const BunchDev &B = bds[tile_bunch[tile]];
-    const int lt = tile - B.tile_begin;
+__device__ __forceinline__ void bin_and_sum(const PassCtx &c, uint32_t *hist, double z,
+                                            double (&v)[4]) {
+    v[0] += z;
+    v[1] += z * z;
+    const int b = hist_bin(z, c.e0, c.e1, c.inv_step, c.nb);
+    if (b >= 0) {
+        atomicAdd(&hist[b], 1u);
+        v[3] += 1.0;
+    }
+}
+
+// One tile of the fused pass.  FX = kStdFlags is the common configuration
+// with the flags folded at compile time; FX = 0 handles any other.  Both
+// evaluate the same expressions, so a bunch gets the same bits from either.
+template <uint32_t FX>
+__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32_t *s_hist,
+                                               double *s_red) {
     PassCtx c;
     load_ctx(B, c);
-    const bool wake = c.flags & F_WAKE;
-    const bool lds_hist = wake && !(c.flags & F_GHIST);
+    const Flags<FX> fl{c.flags};
+    const bool wake = fl.has(F_WAKE);
+    const bool lds_hist = wake && !fl.has(F_GHIST);
     uint32_t *hist = lds_hist ? s_hist : B.hist;
     if (lds_hist) {
         for (int b = threadIdx.x; b < c.nb; b += blockDim.x) s_hist[b] = 0u;
@@ -286,62 +350,96 @@ __global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__re
     const int64_t n = B.n;
     const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
     const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
+    const int64_t full = n >> 1;                    // pairs with two particles
+    const int64_t pf = min(p1, full);
     const TurnScalars &sc = *B.sc;
     const uint32_t tabs = (uint32_t)(B.turn_offset + sc.turn);
-    const double *nrm = (c.flags & F_INJ) ? B.normals + sc.turn * n : nullptr;
+    const bool inj = fl.has(F_INJ);
+    const bool qe = fl.has(F_QE);
+    const double *nrm = inj ? B.normals + sc.turn * n : nullptr;
+    const uint64_t seed = B.seed;
+    const bool kick2 = c.pending_kick && c.nb >= 2;
+    const double2 *__restrict__ P2 = reinterpret_cast<const double2 *>(B.P2);
     double *__restrict__ zp = B.z;
     double *__restrict__ ep = B.dE;
     double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-        const int64_t i = 2 * p;
-        const bool two = (i + 1 < n);
-        double z0, z1 = 0.0, e0, e1 = 0.0;
-        if (two) {
-            const double2 zz = *reinterpret_cast<const double2 *>(zp + i);
-            const double2 ee = *reinterpret_cast<const double2 *>(ep + i);
-            z0 = zz.x; z1 = zz.y; e0 = ee.x; e1 = ee.y;
-        } else {
-            z0 = zp[i];
-            e0 = ep[i];
+
+    int64_t p = p0 + threadIdx.x;
+    double2 zn = make_double2(0.0, 0.0), en = make_double2(0.0, 0.0);
+    if (p < pf) {
+        zn = *reinterpret_cast<const double2 *>(zp + 2 * p);
+        en = *reinterpret_cast<const double2 *>(ep + 2 * p);
+    }
+    for (; p < pf; p += blockDim.x) {
+        const double2 zz = zn, ee = en;
+        const int64_t pn = p + blockDim.x;
+        if (pn < pf) {   // prefetch the next pair
+            zn = *reinterpret_cast<const double2 *>(zp + 2 * pn);
+            en = *reinterpret_cast<const double2 *>(ep + 2 * pn);
+        }
+        // gathers of the previous turn's potential first, the RNG work hides
+        // them; (P[k], P[k+1]) come as one 16-byte load
+        int k0 = 0, k1 = 0;
+        double w0 = 0.0, w1 = 0.0;
+        double2 q0 = make_double2(0.0, 0.0), q1 = make_double2(0.0, 0.0);
+        if (kick2) {
+            interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
+            interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
+            q0 = P2[k0];
+            q1 = P2[k1];
         }
         double r0 = 0.0, r1 = 0.0;
-        if (c.flags & F_QE) {
-            if (nrm) {
-                r0 = nrm[i];
-                if (two) r1 = nrm[i + 1];
+        if (qe) {
+            if (inj) {   // turn rows of an odd bunch are only 8-byte aligned
+                r0 = nrm[2 * p];
+                r1 = nrm[2 * p + 1];
             } else {
-                qe_pair_normals(B.seed, (uint64_t)p, tabs, r0, r1);
+                qe_pair_normals(seed, (uint64_t)p, tabs, r0, r1);
             }
         }
-        double d0, d1 = 0.0;
-        e0 = apply_pending(c, z0, e0);
-        advance(c, z0, e0, r0, d0);
-        if (two) {
-            e1 = apply_pending(c, z1, e1);
-            advance(c, z1, e1, r1, d1);
+        double z0 = zz.x, z1 = zz.y, e0 = ee.x, e1 = ee.y;
+        if (c.pending_kick) {
+            if (kick2) {
+                e0 = e0 - (q0.x * (1.0 - w0) + q0.y * w0);
+                e1 = e1 - (q1.x * (1.0 - w1) + q1.y * w1);
+            } else {
+                e0 = e0 - c.P[0];
+                e1 = e1 - c.P[0];
+            }
         }
+        if (c.pending_m) {
+            e0 = e0 - c.m_prev;
+            e1 = e1 - c.m_prev;
+        }
+        double d0, d1;
+        advance(c, fl, z0, e0, r0, d0);
+        advance(c, fl, z1, e1, r1, d1);
         if (wake) {
-            v[0] += z0;
-            v[1] += z0 * z0;
-            const int b0 = hist_bin(z0, c.e0, c.e1, c.inv_step, c.nb);
-            if (b0 >= 0) { atomicAdd(&hist[b0], 1u); v[3] += 1.0; }
-            if (two) {
-                v[0] += z1;
-                v[1] += z1 * z1;
-                const int b1 = hist_bin(z1, c.e0, c.e1, c.inv_step, c.nb);
-                if (b1 >= 0) { atomicAdd(&hist[b1], 1u); v[3] += 1.0; }
-            }
+            bin_and_sum(c, hist, z0, v);
+            bin_and_sum(c, hist, z1, v);
         }
         v[2] += d0;
-        if (two) v[2] += d1;
-        if (two) {
-            *reinterpret_cast<double2 *>(zp + i) = make_double2(z0, z1);
-            *reinterpret_cast<double2 *>(ep + i) = make_double2(e0, e1);
-        } else {
-            zp[i] = z0;
-            ep[i] = e0;
-        }
+        v[2] += d1;
+        *reinterpret_cast<double2 *>(zp + 2 * p) = make_double2(z0, z1);
+        *reinterpret_cast<double2 *>(ep + 2 * p) = make_double2(e0, e1);
     }
+    // the single particle of an odd bunch (last pair)
+    if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1) {
+        const int64_t i = n - 1;
+        double z0 = zp[i], e0 = ep[i], r0 = 0.0, r1 = 0.0;
+        if (qe) {
+            if (inj) r0 = nrm[i];
+            else qe_pair_normals(seed, (uint64_t)full, tabs, r0, r1);
+        }
+        e0 = apply_pending(c, z0, e0);
+        double d0;
+        advance(c, fl, z0, e0, r0, d0);
+        if (wake) bin_and_sum(c, hist, z0, v);
+        v[2] += d0;
+        zp[i] = z0;
+        ep[i] = e0;
+    }
+    if (!wake) { v[0] = 0.0; v[1] = 0.0; }
     block_sum<4>(v, s_red);
     if (threadIdx.x == 0) {
         double *o = B.part + 4 * (int64_t)lt;
@@ -356,23 +454,47 @@ __global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__re
     }
 }
 
+// The fused pass over a tile table: k_turn_pass_std when every tile's bunch
+// has exactly the standard flags (kStdFlags, LDS histogram), else the
+// generic k_turn_pass.
+template <uint32_t FX>
+__device__ __forceinline__ void turn_pass_entry(const BunchDev *__restrict__ bds,
+                                                const int32_t *__restrict__ tile_bunch) {
+    extern __shared__ uint32_t s_hist[];
+    __shared__ double s_red[4 * 16];
+    const int tile = blockIdx.x;
+    const BunchDev &B = bds[tile_bunch[tile]];
+    turn_pass_tile<FX>(B, tile - B.tile_begin, s_hist, s_red);
+}
+
+__global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__restrict__ bds,
+                                                            const int32_t *__restrict__ tile_bunch) {
+    turn_pass_entry<0>(bds, tile_bunch);
+}
+
+__global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
+    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch) {
+    turn_pass_entry<kStdFlags>(bds, tile_bunch);
+}
+
 // ---------------------------------------------------------------------------
 // bin phase: sigma_z, current, line density, blocked FFT convolution
 // ---------------------------------------------------------------------------
 //
 // P_k = I * sum_{j<=k} W_j g_{k-j},  g_i = lam_i * (cnt_i / n_h).
 // With block size B and F = 2B, output block o is
-//   Re(IFFT(Y))[0,B) + Im(IFFT(Y))[B,2B),  Y = sum_j FFT(g_{o-j} + i g_{o-1-j}) * FFT(W_j),
+//   Re(IFFT(Y_o))[0,B) + Im(IFFT(Y_o))[B,2B),
+//   Y_o = sum_j X_{o-j} FFT(W_j),   X_a = FFT(g_a + i g_{a-1}),
 // since each block product g_a * W_j is real, has length 2B-1 <= F (no
 // wrap), and lands on output blocks a+j (first half) and a+j+1 (second half).
 
-__global__ __launch_bounds__(kConvThreads) void k_conv(const BunchDev *__restrict__ bds,
-                                                       const int32_t *__restrict__ items) {
-    extern __shared__ double2 sX[];
-    __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[3];
-    const BunchDev &B = bds[items[2 * blockIdx.x]];
-    const int o = items[2 * blockIdx.x + 1];
+struct BinScalars {
+    double sz, cur, nh;
+};
+
+// sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386) and the
+// binned count from the tile sums; valid in every thread
+__device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, double *s_bc) {
     const TurnScalars &sc = *B.sc;
     double v[3] = {0.0, 0.0, 0.0};
     for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
@@ -382,67 +504,39 @@ __global__ __launch_bounds__(kConvThreads) void k_conv(const BunchDev *__restric
     }
     block_sum<3>(v, s_red);
     if (threadIdx.x == 0) {
-        const double sz = gstd_from_sums(v[0], v[1], (double)B.n);   // evolution.jl:357
-        const double cur = (B.flags & F_CHARGE)
-                               ? B.charge
-                               : (((B.pf / sc.E0) / (kTwoPi * B.radius)) * sz) /
-                                     (sc.eta0 * (B.sigma_E0 * B.sigma_E0));  // evolution.jl:386
+        const double sz = gstd_from_sums(v[0], v[1], (double)B.n);
         s_bc[0] = sz;
-        s_bc[1] = cur;
+        s_bc[1] = (B.flags & F_CHARGE)
+                      ? B.charge
+                      : (((B.pf / sc.E0) / (kTwoPi * B.radius)) * sz) /
+                            (sc.eta0 * (B.sigma_E0 * B.sigma_E0));
         s_bc[2] = v[2];
-        if (o == 0) {
-            B.sc->sigma_z = sz;
-            B.sc->current = cur;
-            B.sc->n_h = v[2];
-        }
     }
     __syncthreads();
-    const double cur = s_bc[1], nh = s_bc[2];
-    const int nb = B.nbins, Bk = B.B, F = B.F, logF = B.logF, nJ = B.nJ, nA = B.nA;
-    if (nh == 0.0) {  // wakefield.jl:305-311: no particles binned -> zero potential
-        for (int m = threadIdx.x; m < Bk; m += blockDim.x)
-            if (o * Bk + m < nb) B.P[o * Bk + m] = 0.0;
-        return;
-    }
-    const double inv_nh = 1.0 / nh;                                 // wakefield.jl:207
-    const double2 *tw = reinterpret_cast<const double2 *>(B.tw);
-    const double2 *What = reinterpret_cast<const double2 *>(B.What);
-    double2 acc[kConvMaxPerThread];
-#pragma unroll
-    for (int q = 0; q < kConvMaxPerThread; ++q) acc[q] = make_double2(0.0, 0.0);
-    for (int j = 0; j < nJ; ++j) {
-        const int a = o - j, bb = o - 1 - j;
-        for (int m = threadIdx.x; m < F; m += blockDim.x) {
-            double re = 0.0, im = 0.0;
-            if (m < Bk) {
-                if (a >= 0 && a < nA) {
-                    const int i = a * Bk + m;
-                    if (i < nb) re = B.lam[i] * ((double)B.hist[i] * inv_nh);  // wakefield.jl:210,238
-                }
-                if (bb >= 0 && bb < nA) {
-                    const int i = bb * Bk + m;
-                    if (i < nb) im = B.lam[i] * ((double)B.hist[i] * inv_nh);
-                }
-            }
-            sX[bitrev(m, logF)] = make_double2(re, im);
+    return BinScalars{s_bc[0], s_bc[1], s_bc[2]};
+}
+
+__device__ __forceinline__ double line_density(const BunchDev &B, int i, double inv_nh) {
+    return B.lam[i] * ((double)B.hist[i] * inv_nh);   // wakefield.jl:210,238
+}
+
+// sX <- bit-reversed g_a + i g_{a-1}, zero-padded to F
+__device__ inline void load_packed(double2 *sX, const BunchDev &B, int a, double inv_nh) {
+    const int nb = B.nbins, Bk = B.B, F = B.F, logF = B.logF;
+    for (int m = threadIdx.x; m < F; m += blockDim.x) {
+        double re = 0.0, im = 0.0;
+        if (m < Bk) {
+            const int i = a * Bk + m;
+            if (i < nb) re = line_density(B, i, inv_nh);
+            if (a >= 1) im = line_density(B, i - Bk, inv_nh);
         }
-        fft_dit_forward(sX, F, logF, tw);
-#pragma unroll
-        for (int q = 0; q < kConvMaxPerThread; ++q) {
-            const int k = threadIdx.x + q * kConvThreads;
-            if (k < F) {
-                const double2 t = cmul(sX[k], What[(int64_t)j * F + k]);
-                acc[q] = make_double2(acc[q].x + t.x, acc[q].y + t.y);
-            }
-        }
-        __syncthreads();
+        sX[bitrev(m, logF)] = make_double2(re, im);
     }
-#pragma unroll
-    for (int q = 0; q < kConvMaxPerThread; ++q) {
-        const int k = threadIdx.x + q * kConvThreads;
-        if (k < F) sX[k] = acc[q];
-    }
-    fft_dif_inverse(sX, F, logF, tw);
+}
+
+// P block o from the unnormalised inverse FFT in LDS
+__device__ inline void store_block(const BunchDev &B, const double2 *sX, int o, double cur) {
+    const int nb = B.nbins, Bk = B.B, F = B.F, logF = B.logF;
     const double scale = cur / (double)F;
     for (int m = threadIdx.x; m < Bk; m += blockDim.x) {
         const int i = o * Bk + m;
@@ -451,6 +545,89 @@ __global__ __launch_bounds__(kConvThreads) void k_conv(const BunchDev *__restric
         const double2 y1 = sX[bitrev(m + Bk, logF)];
         B.P[i] = (y0.x + y1.y) * scale;
     }
+}
+
+__device__ inline void zero_block(const BunchDev &B, int o) {
+    const int nb = B.nbins, Bk = B.B;
+    for (int m = threadIdx.x; m < Bk; m += blockDim.x)
+        if (o * Bk + m < nb) B.P[o * Bk + m] = 0.0;
+}
+
+// one workgroup per output block, wake support within one block (nJ <= 1)
+__global__ __launch_bounds__(kConvThreads) void k_conv_fused(const BunchDev *__restrict__ bds,
+                                                             const int32_t *__restrict__ items) {
+    extern __shared__ double2 sX[];
+    __shared__ double s_red[3 * 16];
+    __shared__ double s_bc[3];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int o = items[2 * blockIdx.x + 1];
+    const BinScalars bs = bin_scalars(B, s_red, s_bc);
+    if (o == 0 && threadIdx.x == 0) {
+        B.sc->sigma_z = bs.sz;
+        B.sc->current = bs.cur;
+        B.sc->n_h = bs.nh;
+    }
+    if (bs.nh == 0.0 || B.nJ == 0) {   // wakefield.jl:305-311
+        zero_block(B, o);
+        return;
+    }
+    const double2 *tw = reinterpret_cast<const double2 *>(B.tw);
+    const double2 *What = reinterpret_cast<const double2 *>(B.What);
+    load_packed(sX, B, o, 1.0 / bs.nh);
+    fft_dit_forward(sX, B.F, B.logF, tw);
+    for (int k = threadIdx.x; k < B.F; k += blockDim.x) sX[k] = cmul(sX[k], What[k]);
+    fft_dif_inverse(sX, B.F, B.logF, tw);
+    store_block(B, sX, o, bs.cur);
+}
+
+// phase A of the blocked convolution: X_a = FFT(g_a + i g_{a-1})
+__global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__restrict__ bds,
+                                                           const int32_t *__restrict__ items) {
+    extern __shared__ double2 sX[];
+    __shared__ double s_red[3 * 16];
+    __shared__ double s_bc[3];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int a = items[2 * blockIdx.x + 1];
+    const BinScalars bs = bin_scalars(B, s_red, s_bc);
+    if (a == 0 && threadIdx.x == 0) {
+        B.sc->sigma_z = bs.sz;
+        B.sc->current = bs.cur;
+        B.sc->n_h = bs.nh;
+    }
+    if (bs.nh == 0.0) return;
+    load_packed(sX, B, a, 1.0 / bs.nh);
+    fft_dit_forward(sX, B.F, B.logF, reinterpret_cast<const double2 *>(B.tw));
+    double2 *X = reinterpret_cast<double2 *>(B.X) + (int64_t)a * B.F;
+    for (int k = threadIdx.x; k < B.F; k += blockDim.x) X[k] = sX[k];
+}
+
+// phase B: Y_o = sum_j X_{o-j} FFT(W_j), inverse FFT, P block o
+__global__ __launch_bounds__(kConvThreads) void k_conv_inv(const BunchDev *__restrict__ bds,
+                                                           const int32_t *__restrict__ items) {
+    extern __shared__ double2 sX[];
+    __shared__ double s_red[3 * 16];
+    __shared__ double s_bc[3];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int o = items[2 * blockIdx.x + 1];
+    const BinScalars bs = bin_scalars(B, s_red, s_bc);
+    if (bs.nh == 0.0) {
+        zero_block(B, o);
+        return;
+    }
+    const int F = B.F;
+    const int jmax = min(o, B.nJ - 1);
+    const double2 *X = reinterpret_cast<const double2 *>(B.X);
+    const double2 *What = reinterpret_cast<const double2 *>(B.What);
+    for (int k = threadIdx.x; k < F; k += blockDim.x) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int j = 0; j <= jmax; ++j) {
+            const double2 t = cmul(X[(int64_t)(o - j) * F + k], What[(int64_t)j * F + k]);
+            acc = make_double2(acc.x + t.x, acc.y + t.y);
+        }
+        sX[k] = acc;
+    }
+    fft_dif_inverse(sX, F, B.logF, reinterpret_cast<const double2 *>(B.tw));
+    store_block(B, sX, o, bs.cur);
 }
 
 // FFT of one wake block per workgroup (set-up of an evolve call)
@@ -478,17 +655,98 @@ __global__ __launch_bounds__(kPassThreads) void k_interp_sum(const BunchDev *__r
     __shared__ double s_red[16];
     const int tile = blockIdx.x;
     const BunchDev &B = bds[tile_bunch[tile]];
-    if (!((B.flags & F_WAKE) && (B.flags & F_E0))) return;
+    if (!((B.flags & F_WAKE) && (B.flags & F_E0)) || (B.flags & F_PLDS)) return;
     const int lt = tile - B.tile_begin;
-    const int64_t i0 = 2 * (int64_t)lt * B.pairs_per_tile;
-    const int64_t i1 = min(i0 + 2 * B.pairs_per_tile, B.n);
+    const int64_t n = B.n;
+    const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
+    const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
+    const int64_t full = n >> 1;
+    const int64_t pf = min(p1, full);
     const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
     const int nb = B.nbins;
+    const double *__restrict__ P = B.P;
+    const double *__restrict__ zp = B.z;
     double v[1] = {0.0};
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
-        v[0] += interp_pot(B.P, B.z[i], c0, c1, inv, nb);
+    if (nb >= 2) {
+        int64_t p = p0 + threadIdx.x;
+        double2 zn = make_double2(0.0, 0.0);
+        if (p < pf) zn = *reinterpret_cast<const double2 *>(zp + 2 * p);
+        for (; p < pf; p += blockDim.x) {
+            const double2 zz = zn;
+            if (p + blockDim.x < pf) zn = *reinterpret_cast<const double2 *>(zp + 2 * (p + blockDim.x));
+            int k0, k1;
+            double w0, w1;
+            interp_index(zz.x, c0, c1, inv, nb, k0, w0);
+            interp_index(zz.y, c0, c1, inv, nb, k1, w1);
+            v[0] += P[k0] * (1.0 - w0) + P[k0 + 1] * w0;
+            v[0] += P[k1] * (1.0 - w1) + P[k1 + 1] * w1;
+        }
+    } else {
+        int64_t p = p0 + threadIdx.x;
+        for (; p < pf; p += blockDim.x) v[0] += P[0] + P[0];
+    }
+    if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1)
+        v[0] += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
     block_sum<1>(v, s_red);
     if (threadIdx.x == 0) B.ipart[lt] = v[0];
+}
+
+// The same sums for bunches whose potential fits in LDS (F_PLDS): each
+// workgroup walks a contiguous range of tiles, loads the table once per bunch
+// it meets, and the per-particle gathers become LDS reads.  The per-tile sums
+// depend on the tile and this kernel's block size only.
+__global__ __launch_bounds__(kInterpThreads) void k_interp_sum_lds(
+    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch, int ntiles) {
+    extern __shared__ double sP[];
+    __shared__ double s_red[16];
+    const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(ntiles, t0 + per);
+    int staged = -1;
+    for (int tile = t0; tile < t1; ++tile) {
+        const int bi = tile_bunch[tile];
+        const BunchDev &B = bds[bi];
+        if (!((B.flags & F_WAKE) && (B.flags & F_E0) && (B.flags & F_PLDS))) continue;
+        const int nb = B.nbins;
+        if (bi != staged) {
+            __syncthreads();
+            for (int k = threadIdx.x; k < nb; k += blockDim.x) sP[k] = B.P[k];
+            __syncthreads();
+            staged = bi;
+        }
+        const int lt = tile - B.tile_begin;
+        const int64_t n = B.n;
+        const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
+        const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
+        const int64_t full = n >> 1;
+        const int64_t pf = min(p1, full);
+        const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
+        const double *__restrict__ zp = B.z;
+        double v[1] = {0.0};
+        if (nb >= 2) {
+            int64_t p = p0 + threadIdx.x;
+            double2 zn = make_double2(0.0, 0.0);
+            if (p < pf) zn = *reinterpret_cast<const double2 *>(zp + 2 * p);
+            for (; p < pf; p += blockDim.x) {
+                const double2 zz = zn;
+                if (p + blockDim.x < pf)
+                    zn = *reinterpret_cast<const double2 *>(zp + 2 * (p + blockDim.x));
+                int k0, k1;
+                double w0, w1;
+                interp_index(zz.x, c0, c1, inv, nb, k0, w0);
+                interp_index(zz.y, c0, c1, inv, nb, k1, w1);
+                v[0] += sP[k0] * (1.0 - w0) + sP[k0 + 1] * w0;
+                v[0] += sP[k1] * (1.0 - w1) + sP[k1 + 1] * w1;
+            }
+        } else {
+            for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) v[0] += sP[0] + sP[0];
+        }
+        if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1)
+            v[0] += interp_pot(sP, zp[n - 1], c0, c1, inv, nb);
+        block_sum<1>(v, s_red);
+        if (threadIdx.x == 0) B.ipart[lt] = v[0];
+        __syncthreads();   // s_red is reused by the next tile
+    }
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__restrict__ bds,
@@ -502,11 +760,12 @@ __global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__rest
     const int64_t i1 = min(i0 + 2 * B.pairs_per_tile, B.n);
     PassCtx c;
     if (mode == 1) load_ctx(B, c);
+    const bool pend = (mode == 1) && (c.pending_kick || c.pending_m);
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const double z = B.z[i];
         double e = B.dE[i];
-        if (mode == 1 && (c.pending_kick || c.pending_m)) {
+        if (pend) {
             e = apply_pending(c, z, e);
             B.dE[i] = e;
         }
@@ -584,8 +843,15 @@ __global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__re
         }
     }
     block_sum<2>(v, s_red);
-    if (fl & F_WAKE)
-        for (int b = threadIdx.x; b < B.nbins; b += blockDim.x) B.hist[b] = 0u;
+    if (fl & F_WAKE) {
+        const int nb = B.nbins;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            B.hist[b] = 0u;
+            // neighbour pairs of this turn's potential for the next pass
+            B.P2[2 * b] = B.P[b];
+            B.P2[2 * b + 1] = (b + 1 < nb) ? B.P[b + 1] : 0.0;
+        }
+    }
     if (threadIdx.x == 0) {
         TurnScalars s = *B.sc;
         double E0 = s.E0;
@@ -614,7 +880,7 @@ __global__ void k_turn_scalars(const BunchDev *__restrict__ bd, double E0, TurnS
 }
 
 // ---------------------------------------------------------------------------
-// stage kernels (unit parity)
+// stage kernels (unit parity); the same device functions as the fused pass
 // ---------------------------------------------------------------------------
 
 __global__ void k_stage_rf(double V, double sphis, double rf, double phis,
@@ -622,7 +888,7 @@ __global__ void k_stage_rf(double V, double sphis, double rf, double phis,
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const double phi = ((-z[i]) * rf) + phis;
-        dE[i] = dE[i] + V * (sin(phi) - sphis);
+        dE[i] = dE[i] + V * (dsin(phi) - sphis);
     }
 }
 
@@ -638,38 +904,23 @@ __global__ void k_stage_sr(double damp, double *__restrict__ dE, int64_t n) {
 }
 
 __global__ void k_stage_drift(int update_eta, const TurnScalars *__restrict__ scp, double mass,
-                              double alpha_c, double two_pi_h, double phis,
-                              double *__restrict__ z, const double *__restrict__ dE, int64_t n) {
+                              double alpha_c, double phis, double *__restrict__ z,
+                              const double *__restrict__ dE, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const TurnScalars &s = *scp;
     PassCtx c = {};
     c.rf = s.rf;
+    c.inv_rf = s.inv_rf;
     c.phis = phis;
-    c.mass = mass;
+    c.inv_mass = 1.0 / mass;
     c.gamma0 = s.gamma0;
     c.alpha_c = alpha_c;
-    c.two_pi_h = two_pi_h;
-    c.denom = s.denom;
+    c.kcoef = s.kcoef;
     c.coeff_c = s.coeff_c;
     c.drift_skip = s.drift_skip;
     c.flags = update_eta ? F_ETA : 0u;
-    double zz = z[i];
-    const double e = dE[i];
-    if (c.flags & F_ETA) {
-        const double dg = e / c.mass;
-        const double gp = c.gamma0 + dg;
-        const double eta = c.alpha_c - 1.0 / (gp * gp);
-        const double coeff = (c.two_pi_h * eta) / c.denom;
-        double phi = -((zz * c.rf) - c.phis);
-        phi = phi + coeff * e;
-        zz = ((-phi) + c.phis) / c.rf;
-    } else if (!c.drift_skip) {
-        double phi = -((zz * c.rf) - c.phis);
-        phi = phi + c.coeff_c * e;
-        zz = ((-phi) + c.phis) / c.rf;
-    }
-    z[i] = zz;
+    z[i] = drift(c, Flags<0>{c.flags}, z[i], dE[i]);
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_stage_hist(const double *__restrict__ z,
@@ -725,6 +976,16 @@ __global__ void k_generate(uint64_t seed, double mu_z, double sz, double mu_E, d
     dE[i] = mu_E + sE * b;
 }
 
+// Plain device copy (16 B per lane per step): the HBM ceiling of a kernel
+// that reads and writes each byte once, and the calibration of the HBM
+// byte counters.
+__global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ src,
+                                                     double2 *__restrict__ dst, int64_t n2) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
@@ -732,14 +993,24 @@ __global__ void k_generate(uint64_t seed, double mu_z, double sz, double mu_E, d
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t configure_kernels(int lds) {
-    hipError_t e;
-    e = hipFuncSetAttribute((const void *)k_turn_pass, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void *)k_conv, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void *)k_fft_wake, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_stage_hist, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    const void *fns[] = {(const void *)k_turn_pass, (const void *)k_turn_pass_std,
+                         (const void *)k_conv_fused,
+                         (const void *)k_conv_fwd, (const void *)k_conv_inv,
+                         (const void *)k_fft_wake, (const void *)k_stage_hist,
+                         (const void *)k_interp_sum_lds};
+    hipError_t last = hipSuccess;
+    for (const void *f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) last = e;
+    }
+    return last;
+}
+
+hipError_t launch_stream_copy(const double *src, double *dst, int64_t n, int blocks, hipStream_t s) {
+    if (n <= 1) return hipSuccess;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s,
+                       reinterpret_cast<const double2 *>(src), reinterpret_cast<double2 *>(dst), n / 2);
+    return hipGetLastError();
 }
 
 hipError_t launch_moments(const BunchDev *bds, const int32_t *tb, int ntiles, int mode,
@@ -763,22 +1034,39 @@ hipError_t launch_init(const BunchDev *bds, const int32_t *list, int nb, hipStre
 }
 
 hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, int ntiles, size_t lds,
-                            int threads, hipStream_t s) {
+                            int std_only, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_turn_pass, dim3(ntiles), dim3(threads), lds, s, bds, tb);
+    if (std_only)
+        hipLaunchKernelGGL(k_turn_pass_std, dim3(ntiles), dim3(kPassThreads), lds, s, bds, tb);
+    else
+        hipLaunchKernelGGL(k_turn_pass, dim3(ntiles), dim3(kPassThreads), lds, s, bds, tb);
     return hipGetLastError();
 }
 
-hipError_t launch_conv(const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
+hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
                        hipStream_t s) {
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_conv, dim3(nitems), dim3(kConvThreads), (size_t)Fmax * 16, s, bds, items);
+    const size_t lds = (size_t)Fmax * 16;
+    if (kind == 0)
+        hipLaunchKernelGGL(k_conv_fused, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
+    else if (kind == 1)
+        hipLaunchKernelGGL(k_conv_fwd, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
+    else
+        hipLaunchKernelGGL(k_conv_inv, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
     return hipGetLastError();
 }
 
 hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, int ntiles, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_interp_sum, dim3(ntiles), dim3(kPassThreads), 0, s, bds, tb);
+    return hipGetLastError();
+}
+
+hipError_t launch_interp_sum_lds(const BunchDev *bds, const int32_t *tb, int ntiles, int grid,
+                                 size_t lds, hipStream_t s) {
+    if (ntiles <= 0 || grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_interp_sum_lds, dim3((unsigned)grid), dim3(kInterpThreads), lds, s, bds, tb,
+                       ntiles);
     return hipGetLastError();
 }
 
@@ -815,11 +1103,10 @@ hipError_t launch_stage_sr(double damp, double *dE, int64_t n, hipStream_t s) {
 }
 
 hipError_t launch_stage_drift(int update_eta, const TurnScalars *sc, double mass, double alpha_c,
-                              double two_pi_h, double phis, double *z, const double *dE,
-                              int64_t n, hipStream_t s) {
+                              double phis, double *z, const double *dE, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_stage_drift, dim3(blocks_for(n, 256)), dim3(256), 0, s, update_eta, sc,
-                       mass, alpha_c, two_pi_h, phis, z, dE, n);
+                       mass, alpha_c, phis, z, dE, n);
     return hipGetLastError();
 }
 

@@ -9,29 +9,38 @@
 
 namespace hm {
 
-constexpr int kPassThreads = 256;      // turn-pass / moments / interp-sum block
+constexpr int kPassThreads = 512;      // turn-pass / moments / interp-sum block
 constexpr int kConvThreads = 256;      // convolution block
 constexpr int kFinalThreads = 256;     // per-bunch finalize block
+constexpr int kInterpThreads = 1024;   // LDS-staged interpolation sums
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (128 KiB of complex FP64)
-constexpr int kConvMaxPerThread = kMaxFFT / kConvThreads;
+constexpr int kAutoFFTBlock = 1024;    // default convolution block (F = 2048)
 
-// allow the dynamic-LDS kernels the device's LDS per block
+enum ConvKind { CONV_FUSED = 0, CONV_FWD = 1, CONV_INV = 2 };
+
+// allow the dynamic-LDS kernels the device's LDS per block (best effort)
 hipError_t configure_kernels(int max_dynamic_lds);
 
 // per-tile sums of z, z^2, dE, dE^2; mode 1 first applies the pending wake
 // kick and re-centre (the epilogue of the last turn)
 hipError_t launch_moments(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int mode,
                           hipStream_t s);
-// per-bunch spreads from the tile sums; mode 0 -> out[0..1], mode 1 -> out[2..4]
+// per-bunch spreads from the tile sums; mode 0 -> out[0..1], mode 1 -> out[2..8]
 hipError_t launch_moments_final(const BunchDev *bds, const int32_t *bunch_list, int nb, int mode,
                                 hipStream_t s);
 hipError_t launch_init(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
+// std_only: every tile's bunch has (flags & kPhysMask) == kStdFlags
 hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
-                            size_t lds_bytes, int threads, hipStream_t s);
-hipError_t launch_conv(const BunchDev *bds, const int32_t *conv_items, int nitems, int Fmax,
+                            size_t lds_bytes, int std_only, hipStream_t s);
+// items are (bunch, block) pairs; Fmax is the largest FFT length among them
+hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
                        hipStream_t s);
 hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
                              hipStream_t s);
+// the F_PLDS bunches' interpolation sums: `grid` workgroups over the tile
+// table, lds_bytes = 8 x the largest bin count among them
+hipError_t launch_interp_sum_lds(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
+                                 int grid, size_t lds_bytes, hipStream_t s);
 hipError_t launch_finalize(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
 // FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
 hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,
@@ -43,8 +52,8 @@ hipError_t launch_stage_rf(double V, double sphis, double rf, double phis, const
 hipError_t launch_stage_qe(double exc, const double *r, double *dE, int64_t n, hipStream_t s);
 hipError_t launch_stage_sr(double damp, double *dE, int64_t n, hipStream_t s);
 hipError_t launch_stage_drift(int update_eta, const TurnScalars *sc_dev, double mass,
-                              double alpha_c, double two_pi_h, double phis, double *z,
-                              const double *dE, int64_t n, hipStream_t s);
+                              double alpha_c, double phis, double *z, const double *dE, int64_t n,
+                              hipStream_t s);
 hipError_t launch_stage_hist(const double *z, int64_t n, double e_first, double e_last,
                              double inv_step, int nb, uint32_t *hist, int use_lds, hipStream_t s);
 hipError_t launch_stage_interp_kick(const double *z, double *dE, int64_t n, const double *P,
@@ -55,5 +64,7 @@ hipError_t launch_generate(uint64_t seed, double mu_z, double sz, double mu_E, d
                            double *z, double *dE, int64_t n, hipStream_t s);
 hipError_t launch_turn_scalars(const BunchDev *bd_dev, double E0, TurnScalars *out,
                                hipStream_t s);
+// device-to-device copy of n doubles (n even) by a plain grid-stride kernel
+hipError_t launch_stream_copy(const double *src, double *dst, int64_t n, int blocks, hipStream_t s);
 
 }  // namespace hm

@@ -97,8 +97,7 @@ hm_status hm_drift(hm_ctx *ctx, int32_t update_eta, double E0, double mass, doub
     double *dz, *de;
     HM_TRY(stage_in(ctx, 0, z, n, &dz));
     HM_TRY(stage_in(ctx, 1, dE, n, &de));
-    HIP_TRY(launch_stage_drift(update_eta, ctx->ssc.p, mass, alpha_c, bd.two_pi_h, phi_s, dz, de, n,
-                               ctx->stream));
+    HIP_TRY(launch_stage_drift(update_eta, ctx->ssc.p, mass, alpha_c, phi_s, dz, de, n, ctx->stream));
     TurnScalars sc;
     HIP_TRY(hipMemcpyAsync(&sc, ctx->ssc.p, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
     HM_TRY(stage_out(ctx, 0, z, n));
@@ -163,15 +162,17 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     d.nA = (int32_t)((nbins + Bk - 1) / Bk);
     d.nJ = (int32_t)((nw + Bk - 1) / Bk);
     const size_t what = (size_t)std::max(d.nJ, 1) * (size_t)d.F * 2;
+    const size_t xs = (size_t)d.nA * (size_t)d.F * 2;
     HM_TRY(ctx->shist.ensure((size_t)nbins));
     HM_TRY(ctx->sbuf[0].ensure((size_t)nbins));   // W
     HM_TRY(ctx->sbuf[1].ensure((size_t)nbins));   // lam
     HM_TRY(ctx->sbuf[2].ensure((size_t)nbins));   // P
-    HM_TRY(ctx->sbuf[3].ensure(what + 8));        // What + tile partials
+    HM_TRY(ctx->sbuf[3].ensure(what + xs + 8));   // What, X, tile partials
     HM_TRY(ctx->ssc.ensure(1));
     HM_TRY(ctx->sbd.ensure(1));
     double *What = ctx->sbuf[3].p;
-    double *part = What + what;
+    double *part = What + what + xs;
+    d.X = What + what;
     const double hp[4] = {0.0, 0.0, 0.0, (double)nh};
     const double *tw;
     HM_TRY(twiddles(ctx, d.F, &tw));
@@ -201,7 +202,12 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     HIP_TRY(hipMemcpyAsync(ctx->sbd.p, &d, sizeof(d), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(di.p, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fft_wake(ctx->sbd.p, ctx->sbuf[0].p, d.nJ, d.F, s));
-    HIP_TRY(launch_conv(ctx->sbd.p, di.p, d.nA, d.F, s));
+    if (d.nJ <= 1) {
+        HIP_TRY(launch_conv(CONV_FUSED, ctx->sbd.p, di.p, d.nA, d.F, s));
+    } else {
+        HIP_TRY(launch_conv(CONV_FWD, ctx->sbd.p, di.p, d.nA, d.F, s));
+        HIP_TRY(launch_conv(CONV_INV, ctx->sbd.p, di.p, d.nA, d.F, s));
+    }
     HIP_TRY(hipMemcpyAsync(P, d.P, (size_t)nbins * 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     di.release();
@@ -242,6 +248,37 @@ hm_status hm_global_std(hm_ctx *ctx, const double *x, int64_t n, double *out) {
     hm_result r;
     HM_TRY(hm_bunch_evolve(b, 0, &p, nullptr, &r));
     *out = r.sigma_z0;
+    return HM_OK;
+}
+
+hm_status hm_stream_copy(hm_ctx *ctx, int64_t bytes, int32_t iters, double *ms_avg, double *gbs) {
+    if (!ctx || bytes < 16 || iters <= 0) return fail(HM_ERR_ARG, "bad arguments");
+    Lock lk(ctx->mu);
+    HM_TRY(set_device(ctx));
+    const int64_t n = (bytes / 16) * 2;   // doubles
+    DevBuf<double> a, b;
+    HM_TRY(a.ensure((size_t)n));
+    HM_TRY(b.ensure((size_t)n));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemsetAsync(a.p, 0, (size_t)n * 8, s));
+    const int blocks = ctx->cus * 8;
+    HIP_TRY(launch_stream_copy(a.p, b.p, n, blocks, s));   // warm-up
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+    for (int k = 0; k < iters; ++k) HIP_TRY(launch_stream_copy(a.p, b.p, n, blocks, s));
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double avg = (double)ms / iters;
+    if (ms_avg) *ms_avg = avg;
+    if (gbs) *gbs = 2.0 * (double)n * 8.0 / (avg * 1e-3) / 1e9;
+    a.release();
+    b.release();
     return HM_OK;
 }
 

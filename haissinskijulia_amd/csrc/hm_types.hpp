@@ -4,8 +4,10 @@
 //   z[n], dE[n]                 particle state, SoA (data_structures.jl:26-29)
 //   hist[nbins]   (uint32)      line-density counts of the current turn
 //   P[nbins]                    wake potential at the bin centres (previous turn)
+//   P2[nbins][2]                the same as neighbour pairs, one 16-byte gather per particle
 //   lam[nbins]                  delta(c_i, 0.15) line-density kernel (wakefield.jl:219-221)
 //   What[nJ][F]   (complex)     FFT of the wake blocks, fixed for one evolve call
+//   X[nA][F]      (complex)     FFT of the packed line-density blocks of a turn
 //   part[ntiles][4]             per-tile sums  (sum z, sum z^2, sum dE'-dE_init, in-range count)
 //   ipart[ntiles]               per-tile sum of the interpolated potential (E0 update)
 //   moments[ntiles][4]          per-tile sums of z, z^2, dE, dE^2 (initial / final spreads)
@@ -29,14 +31,22 @@ enum BunchFlags : uint32_t {
     F_INJ = 32u,       // injected normals instead of Philox
     F_CHARGE = 64u,    // fixed wake current (charge extension / stage entry)
     F_GHIST = 128u,    // histogram straight to global memory (too many bins for LDS)
+    F_PLDS = 256u,     // the potential fits in LDS: interpolation sums read it there
 };
+
+// the all-physics-on configuration the turn pass is specialised for, and the
+// flags that select the pass's code path
+constexpr uint32_t kStdFlags = F_WAKE | F_ETA | F_E0 | F_SR | F_QE;
+constexpr uint32_t kPhysMask = F_WAKE | F_ETA | F_E0 | F_SR | F_QE | F_INJ | F_GHIST;
 
 // Scalars of one turn of one bunch (evolution.jl:300-304 plus the pending
 // state of the fused pass).  Written by k_init / k_finalize, read by the
 // per-particle kernels.
 struct TurnScalars {
     double E0, gamma0, beta0, eta0, rf;
+    double inv_rf;     // 1/rf (the phase-to-z conversion, utils.jl:476)
     double denom;      // (beta0*beta0)*E0, drift denominator (evolution.jl:338)
+    double kcoef;      // (2 pi h)/denom: the phase coefficient is eta * kcoef
     double coeff_c;    // constant-eta phase coefficient (evolution.jl:157)
     double damp;       // 1 - dU/dE (synchrotron_radiation.jl:44-45)
     double exc;        // sqrt(1-(1-dU/dE)^2) * sigma_E0 (quantum_excitation.jl:58-59)
@@ -58,7 +68,7 @@ struct BunchDev {
     uint32_t flags;
     int32_t nbins;
     // physics constants of the call
-    double mass, voltage, alpha_c, phis, sin_phis, radius, freq_rf, two_pi_h;
+    double mass, inv_mass, voltage, alpha_c, phis, sin_phis, radius, freq_rf, two_pi_h;
     double sigma_E0;   // initial energy spread (drives QE and the wake current)
     double pf;         // particle factor of evolution.jl:377-383 (depends on n only)
     double charge;     // fixed current when F_CHARGE
@@ -71,9 +81,11 @@ struct BunchDev {
     double c_first, c_last, inv_s;
     uint32_t *hist;
     double *P;
+    double *P2;           // interleaved pairs (P[k], P[k+1]), k < nbins (P[nbins] := 0)
     const double *lam;
     const double *What;   // interleaved complex [nJ][F][2]
     const double *tw;     // interleaved complex [F/2][2], exp(-2 pi i k / F)
+    double *X;            // interleaved complex [nA][F][2]: FFT(g_a + i g_{a-1}) (nJ > 1)
     int32_t B, F, logF, nA, nJ, pad_;
     double *part, *ipart, *moments, *out;
     TurnScalars *sc;

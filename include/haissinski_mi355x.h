@@ -158,6 +158,12 @@ hm_status hm_normals(hm_ctx *ctx, uint64_t seed, int64_t turn, int64_t i0, int64
  * correctly rounded; host-only helper, needs no GPU */
 hm_status hm_bin_grid(double zw, int64_t nbins, double *edges, double *centers, double *step);
 
+/* ---- measurement -------------------------------------------------------- */
+/* No counterpart in the reference: the HBM ceiling of this GPU.  Copies
+ * `bytes` (rounded down to 16) device-to-device with a plain kernel `iters`
+ * times and reports the average kernel time and read+write bandwidth. */
+hm_status hm_stream_copy(hm_ctx *ctx, int64_t bytes, int32_t iters, double *ms_avg, double *gb_per_s);
+
 /* ---- serving: continuous batching of tracking jobs ---------------------- */
 typedef struct hm_server hm_server;
 typedef struct {

@@ -56,7 +56,10 @@ def test_quantum_excitation_and_sr_bit_exact(dev):
 
 
 @pytest.mark.parametrize("update_eta", [True, False])
-def test_drift_bit_exact(dev, update_eta):
+def test_drift_within_one_phase_ulp(dev, update_eta):
+    # The kernel multiplies by 1/rf, 1/mass and (2 pi h)/denom where the
+    # reference divides; the phase phi then rounds to the same double or a
+    # neighbour, so z agrees to one ulp of phi_s / rf plus a few ulp of z.
     p = W.ring_3gev_lattice(flags=(True, update_eta, True, True, True))
     z, dE = _bunch(5000)
     E0 = p["E0"]
@@ -71,7 +74,8 @@ def test_drift_bit_exact(dev, update_eta):
     part = H.Particles(z.copy(), dE)
     skipped = H.drift(part, H.SimulationParameters.from_dict(p), E0, dev)
     assert not skipped
-    np.testing.assert_array_equal(part.coordinates.z, ref)
+    bound = 2 * np.spacing(p["phi_s"]) / rf + 4 * np.spacing(np.abs(ref))
+    assert np.all(np.abs(part.coordinates.z - ref) <= bound)
 
 
 def test_drift_skip_condition(dev):
