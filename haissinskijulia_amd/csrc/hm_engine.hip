@@ -492,7 +492,7 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         if ((size_t)nb * 4 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
         if ((size_t)nb * 8 <= (size_t)ctx_->lds_max - 1024) d.flags |= F_PLDS;
         Work &w = q.b->w;
-        HM_TRY(w.hist.ensure((size_t)nb));
+        HM_TRY(w.hist.ensure((size_t)nb * 2));
         HM_TRY(w.P.ensure((size_t)nb));
         HM_TRY(w.P2.ensure((size_t)nb * 2));
         HM_TRY(w.lam.ensure((size_t)nb));

@@ -14,7 +14,7 @@
 //   k_interp_sum  (update_E0 with the wake) sum of the interpolated potential
 //                 over the post-drift z, which the E0 update of
 //                 evolution.jl:404-435 needs before the next turn.
-//   k_finalize    E0 update, scalars of the next turn, histogram reset.
+//   k_finalize    E0 update and the scalars of the next turn.
 //
 // All floating-point work is FP64.  Multiplies and adds are never contracted
 // (-ffp-contract=off) except inside the elementary functions of hm_math.hpp;
@@ -25,7 +25,17 @@
 
 #include "hm_kernels.hpp"
 
+// a*b + k with the polynomial constant k in scalar registers.  Written as
+// plain fma, the compiler keeps the ~30 constants of the pass in vector
+// registers and copies one into the accumulator before every v_fmac; this
+// form needs no copy and leaves the vector registers to the particles.
+__device__ __forceinline__ double fma_sk(double a, double b, double k) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(k));
+    return d;
+}
 #define HM_QUAL __device__ __forceinline__
+#define HM_FMAK(a, b, k) ::fma_sk(a, b, k)
 #include "hm_math.hpp"
 
 namespace hm {
@@ -114,6 +124,9 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
         c2 = n2;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
+        // the round keys are uniform: recomputed per round by scalar adds
+        // rather than twenty of them held (and spilled) across the pass
+        asm volatile("" : "+s"(k0), "+s"(k1));
     }
     x[0] = c0; x[1] = c1; x[2] = c2; x[3] = c3;
 }
@@ -156,6 +169,11 @@ __device__ __forceinline__ double interp_pot(const double *__restrict__ P, doubl
     double w;
     interp_index(z, c0, c1, inv, nb, k, w);
     return P[k] * (1.0 - w) + P[k + 1] * w;
+}
+
+// this turn's half of the two-turn histogram buffer
+__device__ __forceinline__ uint32_t *hist_of(const BunchDev &B, int64_t turn) {
+    return B.hist + (turn & 1) * (int64_t)B.nbins;
 }
 
 // histogram bin of calculate_local_histogram! (utils.jl:387-397); -1 = not counted
@@ -342,7 +360,9 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
     const Flags<FX> fl{c.flags};
     const bool wake = fl.has(F_WAKE);
     const bool lds_hist = wake && !fl.has(F_GHIST);
-    uint32_t *hist = lds_hist ? s_hist : B.hist;
+    const TurnScalars &sc = *B.sc;
+    uint32_t *ghist = hist_of(B, sc.turn);
+    uint32_t *hist = lds_hist ? s_hist : ghist;
     if (lds_hist) {
         for (int b = threadIdx.x; b < c.nb; b += blockDim.x) s_hist[b] = 0u;
         __syncthreads();
@@ -352,7 +372,6 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
     const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
     const int64_t full = n >> 1;                    // pairs with two particles
     const int64_t pf = min(p1, full);
-    const TurnScalars &sc = *B.sc;
     const uint32_t tabs = (uint32_t)(B.turn_offset + sc.turn);
     const bool inj = fl.has(F_INJ);
     const bool qe = fl.has(F_QE);
@@ -449,7 +468,7 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
         __syncthreads();
         for (int b = threadIdx.x; b < c.nb; b += blockDim.x) {
             const uint32_t h = s_hist[b];
-            if (h) atomicAdd(&B.hist[b], h);
+            if (h) atomicAdd(&ghist[b], h);
         }
     }
 }
@@ -516,41 +535,62 @@ __device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, doubl
     return BinScalars{s_bc[0], s_bc[1], s_bc[2]};
 }
 
-__device__ __forceinline__ double line_density(const BunchDev &B, int i, double inv_nh) {
-    return B.lam[i] * ((double)B.hist[i] * inv_nh);   // wakefield.jl:210,238
+__device__ __forceinline__ double line_density(const BunchDev &B, const uint32_t *h, int i,
+                                               double inv_nh) {
+    return B.lam[i] * ((double)h[i] * inv_nh);   // wakefield.jl:210,238
 }
 
 // sX <- bit-reversed g_a + i g_{a-1}, zero-padded to F
 __device__ inline void load_packed(double2 *sX, const BunchDev &B, int a, double inv_nh) {
     const int nb = B.nbins, Bk = B.B, F = B.F, logF = B.logF;
+    const uint32_t *h = hist_of(B, B.sc->turn);
     for (int m = threadIdx.x; m < F; m += blockDim.x) {
         double re = 0.0, im = 0.0;
         if (m < Bk) {
             const int i = a * Bk + m;
-            if (i < nb) re = line_density(B, i, inv_nh);
-            if (a >= 1) im = line_density(B, i - Bk, inv_nh);
+            if (i < nb) re = line_density(B, h, i, inv_nh);
+            if (a >= 1) im = line_density(B, h, i - Bk, inv_nh);
         }
         sX[bitrev(m, logF)] = make_double2(re, im);
     }
 }
 
+// clears bins [aB, aB+B) of the histogram half the next turn fills (nothing
+// reads it during this turn)
+__device__ inline void clear_next_hist(const BunchDev &B, int a) {
+    const int nb = B.nbins, Bk = B.B;
+    uint32_t *h = hist_of(B, B.sc->turn + 1);
+    for (int m = threadIdx.x; m < Bk; m += blockDim.x)
+        if (a * Bk + m < nb) h[a * Bk + m] = 0u;
+}
+
+// writes P[i] and its two places in the neighbour-pair table,
+// P2[i].x = P[i] and P2[i-1].y = P[i] (P2[nb-1].y stays 0)
+__device__ __forceinline__ void put_potential(const BunchDev &B, int i, double v) {
+    B.P[i] = v;
+    if (B.P2) {
+        B.P2[2 * i] = v;
+        if (i >= 1) B.P2[2 * i - 1] = v;
+    }
+}
+
 // P block o from the unnormalised inverse FFT in LDS
 __device__ inline void store_block(const BunchDev &B, const double2 *sX, int o, double cur) {
-    const int nb = B.nbins, Bk = B.B, F = B.F, logF = B.logF;
-    const double scale = cur / (double)F;
+    const int nb = B.nbins, Bk = B.B, logF = B.logF;
+    const double scale = cur / (double)B.F;
     for (int m = threadIdx.x; m < Bk; m += blockDim.x) {
         const int i = o * Bk + m;
         if (i >= nb) continue;
         const double2 y0 = sX[bitrev(m, logF)];
         const double2 y1 = sX[bitrev(m + Bk, logF)];
-        B.P[i] = (y0.x + y1.y) * scale;
+        put_potential(B, i, (y0.x + y1.y) * scale);
     }
 }
 
 __device__ inline void zero_block(const BunchDev &B, int o) {
     const int nb = B.nbins, Bk = B.B;
     for (int m = threadIdx.x; m < Bk; m += blockDim.x)
-        if (o * Bk + m < nb) B.P[o * Bk + m] = 0.0;
+        if (o * Bk + m < nb) put_potential(B, o * Bk + m, 0.0);
 }
 
 // one workgroup per output block, wake support within one block (nJ <= 1)
@@ -567,6 +607,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fused(const BunchDev *__r
         B.sc->current = bs.cur;
         B.sc->n_h = bs.nh;
     }
+    clear_next_hist(B, o);
     if (bs.nh == 0.0 || B.nJ == 0) {   // wakefield.jl:305-311
         zero_block(B, o);
         return;
@@ -594,6 +635,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
         B.sc->current = bs.cur;
         B.sc->n_h = bs.nh;
     }
+    clear_next_hist(B, a);
     if (bs.nh == 0.0) return;
     load_packed(sX, B, a, 1.0 / bs.nh);
     fft_dit_forward(sX, B.F, B.logF, reinterpret_cast<const double2 *>(B.tw));
@@ -601,9 +643,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
     for (int k = threadIdx.x; k < B.F; k += blockDim.x) X[k] = sX[k];
 }
 
-// phase B: Y_o = sum_j X_{o-j} FFT(W_j), inverse FFT, P block o
-__global__ __launch_bounds__(kConvThreads) void k_conv_inv(const BunchDev *__restrict__ bds,
-                                                           const int32_t *__restrict__ items) {
+// phase B: Y_o = sum_j X_{o-j} FFT(W_j), inverse FFT, P block o.  The
+// products are summed in j order; four j at a time have their loads in flight.
+__global__ __launch_bounds__(kConvInvThreads) void k_conv_inv(const BunchDev *__restrict__ bds,
+                                                              const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
     __shared__ double s_red[3 * 16];
     __shared__ double s_bc[3];
@@ -620,7 +663,21 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_inv(const BunchDev *__res
     const double2 *What = reinterpret_cast<const double2 *>(B.What);
     for (int k = threadIdx.x; k < F; k += blockDim.x) {
         double2 acc = make_double2(0.0, 0.0);
-        for (int j = 0; j <= jmax; ++j) {
+        int j = 0;
+        for (; j + 4 <= jmax + 1; j += 4) {
+            double2 xa[4], wa[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                xa[u] = X[(int64_t)(o - j - u) * F + k];
+                wa[u] = What[(int64_t)(j + u) * F + k];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double2 t = cmul(xa[u], wa[u]);
+                acc = make_double2(acc.x + t.x, acc.y + t.y);
+            }
+        }
+        for (; j <= jmax; ++j) {
             const double2 t = cmul(X[(int64_t)(o - j) * F + k], What[(int64_t)j * F + k]);
             acc = make_double2(acc.x + t.x, acc.y + t.y);
         }
@@ -817,7 +874,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restri
                                                         const int32_t *__restrict__ list) {
     const BunchDev &B = bds[list[blockIdx.x]];
     if (B.flags & F_WAKE)
-        for (int b = threadIdx.x; b < B.nbins; b += blockDim.x) B.hist[b] = 0u;
+        for (int b = threadIdx.x; b < 2 * B.nbins; b += blockDim.x) B.hist[b] = 0u;
     if (threadIdx.x == 0) {
         TurnScalars s = {};
         turn_scalars(B, B.E0_init, s);
@@ -843,15 +900,6 @@ __global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__re
         }
     }
     block_sum<2>(v, s_red);
-    if (fl & F_WAKE) {
-        const int nb = B.nbins;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            B.hist[b] = 0u;
-            // neighbour pairs of this turn's potential for the next pass
-            B.P2[2 * b] = B.P[b];
-            B.P2[2 * b + 1] = (b + 1 < nb) ? B.P[b + 1] : 0.0;
-        }
-    }
     if (threadIdx.x == 0) {
         TurnScalars s = *B.sc;
         double E0 = s.E0;
@@ -1052,7 +1100,7 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
     else if (kind == 1)
         hipLaunchKernelGGL(k_conv_fwd, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
     else
-        hipLaunchKernelGGL(k_conv_inv, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
+        hipLaunchKernelGGL(k_conv_inv, dim3(nitems), dim3(kConvInvThreads), lds, s, bds, items);
     return hipGetLastError();
 }
 

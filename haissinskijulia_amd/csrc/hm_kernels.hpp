@@ -11,6 +11,7 @@ namespace hm {
 
 constexpr int kPassThreads = 512;      // turn-pass / moments / interp-sum block
 constexpr int kConvThreads = 256;      // convolution block
+constexpr int kConvInvThreads = 1024;  // phase-B convolution block (spectral sums in flight)
 constexpr int kFinalThreads = 256;     // per-bunch finalize block
 constexpr int kInterpThreads = 1024;   // LDS-staged interpolation sums
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (128 KiB of complex FP64)

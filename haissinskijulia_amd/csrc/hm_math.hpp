@@ -8,11 +8,16 @@
 // The includer defines HM_QUAL (the function qualifier) and an Ops policy
 // with rcp(a) (an estimate of 1/a), frexp_mant(u) in [0.5, 1) and
 // frexp_exp(u), so one source serves the gfx950 kernels and the host test.
+// HM_FMAK(a, b, k) is a*b + k with one rounding for a constant k; the
+// kernels define it to keep k in scalar registers.
 #pragma once
 #include <stdint.h>
 
 #ifndef HM_QUAL
 #define HM_QUAL inline
+#endif
+#ifndef HM_FMAK
+#define HM_FMAK(a, b, k) __builtin_fma(a, b, k)
 #endif
 
 namespace hm {
@@ -33,13 +38,13 @@ constexpr double kSqrtHalf = 0.70710678118654752440;
 HM_QUAL double sin_poly(double r) {
     const double r2 = r * r;
     double p = -1.0 / 355687428096000.0;               // -1/17!
-    p = __builtin_fma(p, r2, 1.0 / 1307674368000.0);   // +1/15!
-    p = __builtin_fma(p, r2, -1.0 / 6227020800.0);     // -1/13!
-    p = __builtin_fma(p, r2, 1.0 / 39916800.0);        // +1/11!
-    p = __builtin_fma(p, r2, -1.0 / 362880.0);         // -1/9!
-    p = __builtin_fma(p, r2, 1.0 / 5040.0);            // +1/7!
-    p = __builtin_fma(p, r2, -1.0 / 120.0);            // -1/5!
-    p = __builtin_fma(p, r2, 1.0 / 6.0);               // +1/3!
+    p = HM_FMAK(p, r2,1.0 / 1307674368000.0);   // +1/15!
+    p = HM_FMAK(p, r2,-1.0 / 6227020800.0);     // -1/13!
+    p = HM_FMAK(p, r2,1.0 / 39916800.0);        // +1/11!
+    p = HM_FMAK(p, r2,-1.0 / 362880.0);         // -1/9!
+    p = HM_FMAK(p, r2,1.0 / 5040.0);            // +1/7!
+    p = HM_FMAK(p, r2,-1.0 / 120.0);            // -1/5!
+    p = HM_FMAK(p, r2,1.0 / 6.0);               // +1/3!
     return __builtin_fma(-(p * r2), r, r);
 }
 
@@ -47,13 +52,13 @@ HM_QUAL double sin_poly(double r) {
 HM_QUAL double cos_poly(double r) {
     const double r2 = r * r;
     double p = -1.0 / 6402373705728000.0;              // -1/18!
-    p = __builtin_fma(p, r2, 1.0 / 20922789888000.0);  // +1/16!
-    p = __builtin_fma(p, r2, -1.0 / 87178291200.0);    // -1/14!
-    p = __builtin_fma(p, r2, 1.0 / 479001600.0);       // +1/12!
-    p = __builtin_fma(p, r2, -1.0 / 3628800.0);        // -1/10!
-    p = __builtin_fma(p, r2, 1.0 / 40320.0);           // +1/8!
-    p = __builtin_fma(p, r2, -1.0 / 720.0);            // -1/6!
-    p = __builtin_fma(p, r2, 1.0 / 24.0);              // +1/4!
+    p = HM_FMAK(p, r2,1.0 / 20922789888000.0);  // +1/16!
+    p = HM_FMAK(p, r2,-1.0 / 87178291200.0);    // -1/14!
+    p = HM_FMAK(p, r2,1.0 / 479001600.0);       // +1/12!
+    p = HM_FMAK(p, r2,-1.0 / 3628800.0);        // -1/10!
+    p = HM_FMAK(p, r2,1.0 / 40320.0);           // +1/8!
+    p = HM_FMAK(p, r2,-1.0 / 720.0);            // -1/6!
+    p = HM_FMAK(p, r2,1.0 / 24.0);              // +1/4!
     // 1 - r^2/2 evaluated as w + (exact remainder) to keep the last bits
     const double hr2 = 0.5 * r2;
     const double w = 1.0 - hr2;
@@ -120,16 +125,16 @@ HM_QUAL double log_pos(double u) {
     const double s2 = s * s;
     // log m = 2 atanh s = 2 s (1 + s^2/3 + s^4/5 + ... + s^22/23)
     double p = 1.0 / 23.0;
-    p = __builtin_fma(p, s2, 1.0 / 21.0);
-    p = __builtin_fma(p, s2, 1.0 / 19.0);
-    p = __builtin_fma(p, s2, 1.0 / 17.0);
-    p = __builtin_fma(p, s2, 1.0 / 15.0);
-    p = __builtin_fma(p, s2, 1.0 / 13.0);
-    p = __builtin_fma(p, s2, 1.0 / 11.0);
-    p = __builtin_fma(p, s2, 1.0 / 9.0);
-    p = __builtin_fma(p, s2, 1.0 / 7.0);
-    p = __builtin_fma(p, s2, 1.0 / 5.0);
-    p = __builtin_fma(p, s2, 1.0 / 3.0);
+    p = HM_FMAK(p, s2,1.0 / 21.0);
+    p = HM_FMAK(p, s2,1.0 / 19.0);
+    p = HM_FMAK(p, s2,1.0 / 17.0);
+    p = HM_FMAK(p, s2,1.0 / 15.0);
+    p = HM_FMAK(p, s2,1.0 / 13.0);
+    p = HM_FMAK(p, s2,1.0 / 11.0);
+    p = HM_FMAK(p, s2,1.0 / 9.0);
+    p = HM_FMAK(p, s2,1.0 / 7.0);
+    p = HM_FMAK(p, s2,1.0 / 5.0);
+    p = HM_FMAK(p, s2,1.0 / 3.0);
     const double t = s + s;
     const double lm = __builtin_fma(t * s2, p, t);
     const double de = (double)e;

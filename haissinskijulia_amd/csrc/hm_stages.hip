@@ -163,7 +163,7 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     d.nJ = (int32_t)((nw + Bk - 1) / Bk);
     const size_t what = (size_t)std::max(d.nJ, 1) * (size_t)d.F * 2;
     const size_t xs = (size_t)d.nA * (size_t)d.F * 2;
-    HM_TRY(ctx->shist.ensure((size_t)nbins));
+    HM_TRY(ctx->shist.ensure((size_t)nbins * 2));
     HM_TRY(ctx->sbuf[0].ensure((size_t)nbins));   // W
     HM_TRY(ctx->sbuf[1].ensure((size_t)nbins));   // lam
     HM_TRY(ctx->sbuf[2].ensure((size_t)nbins));   // P
@@ -176,7 +176,7 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     const double hp[4] = {0.0, 0.0, 0.0, (double)nh};
     const double *tw;
     HM_TRY(twiddles(ctx, d.F, &tw));
-    d.hist = ctx->shist.p;
+    d.hist = ctx->shist.p;   // counts in the first half (turn 0); the kernels clear the second
     d.lam = ctx->sbuf[1].p;
     d.P = ctx->sbuf[2].p;
     d.What = What;

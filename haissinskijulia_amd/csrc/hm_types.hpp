@@ -2,7 +2,8 @@
 //
 // Device memory layout of one bunch (all FP64 unless noted):
 //   z[n], dE[n]                 particle state, SoA (data_structures.jl:26-29)
-//   hist[nbins]   (uint32)      line-density counts of the current turn
+//   hist[2][nbins] (uint32)     line-density counts; turn t fills half t&1 while
+//                               the bin phase clears the other half
 //   P[nbins]                    wake potential at the bin centres (previous turn)
 //   P2[nbins][2]                the same as neighbour pairs, one 16-byte gather per particle
 //   lam[nbins]                  delta(c_i, 0.15) line-density kernel (wakefield.jl:219-221)
@@ -79,7 +80,7 @@ struct BunchDev {
     // histogram / interpolation grids (evolution.jl:270-274, wakefield.jl:38-42)
     double e_first, e_last, inv_step;
     double c_first, c_last, inv_s;
-    uint32_t *hist;
+    uint32_t *hist;       // [2][nbins]: half (turn & 1) is this turn's
     double *P;
     double *P2;           // interleaved pairs (P[k], P[k+1]), k < nbins (P[nbins] := 0)
     const double *lam;
