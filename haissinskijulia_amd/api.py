@@ -338,6 +338,103 @@ def batch_evolve(bunches: Sequence[DeviceBunch], nbins: Sequence[int],
     return [EvolveResult._from(r) for r in rs]
 
 
+@dataclass
+class Job:
+    """One tracking job for ``Server``: a Gaussian bunch generated on the GPU
+    from ``particle_seed`` (``DeviceBunch.generate``), then ``params.n_turns``
+    turns."""
+    params: SimulationParameters
+    n: int
+    nbins: int
+    particle_seed: int
+    sigma_z: float
+    sigma_E: float
+    options: TrackingOptions = field(default_factory=TrackingOptions)
+    fail_after_turns: int = 0   # >0: stop the job early with HM_ERR_CANCELLED (failure hook)
+
+
+@dataclass
+class JobResult:
+    job_id: int
+    status: int            # 0, or HM_ERR_CANCELLED for cancelled / failed jobs
+    result: EvolveResult
+    t_submit: float        # server clock [s]
+    t_admit: float
+    t_done: float
+    centroid_z: float
+    centroid_E: float
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_submit
+
+
+class Server:
+    """Continuous batching of tracking jobs on one GPU (hm_server_*).
+
+    ``submit`` queues a job; each ``tick(turns)`` admits queued jobs while the
+    HBM budget and slot count allow, advances every resident job by up to
+    ``turns`` turns in shared launches and retires finished ones; ``poll``
+    returns finished results in completion order.  A job's result does not
+    depend on what else is resident."""
+
+    def __init__(self, budget_bytes: int, max_slots: int, device: Optional[MI355X] = None):
+        self.device = device or MI355X.default()
+        h = C.c_void_p()
+        check(_lib.load().hm_server_create(self.device.ctx, int(budget_bytes), int(max_slots),
+                                           C.byref(h)))
+        self._h = h
+
+    def submit(self, job: Job) -> int:
+        keep: list = []
+        j = _lib.HmJob()
+        j.params = job.params._c()
+        j.opts = job.options._c(keep, job.n, job.params.n_turns, job.nbins)
+        j.n = int(job.n)
+        j.nbins = int(job.nbins)
+        j.particle_seed = int(job.particle_seed) & 0xFFFFFFFFFFFFFFFF
+        j.sigma_z = float(job.sigma_z)
+        j.sigma_E = float(job.sigma_E)
+        j.fail_after_turns = int(job.fail_after_turns)
+        jid = C.c_int64()
+        check(_lib.load().hm_server_submit(self._h, C.byref(j), C.byref(jid)))
+        return jid.value
+
+    def cancel(self, job_id: int) -> None:
+        check(_lib.load().hm_server_cancel(self._h, int(job_id)))
+
+    def tick(self, turns: int) -> int:
+        """Advance resident jobs; returns the number still queued or running."""
+        act = C.c_int32()
+        check(_lib.load().hm_server_tick(self._h, int(turns), C.byref(act)))
+        return act.value
+
+    def poll(self, cap: int = 256) -> list[JobResult]:
+        buf = (_lib.HmJobResult * cap)()
+        cnt = C.c_int32()
+        check(_lib.load().hm_server_poll(self._h, buf, cap, C.byref(cnt)))
+        return [JobResult(r.job_id, r.status, EvolveResult._from(r.result), r.t_submit, r.t_admit,
+                          r.t_done, r.centroid_z, r.centroid_E) for r in buf[:cnt.value]]
+
+    def run(self, turns_per_tick: int = 100) -> list[JobResult]:
+        """Tick until every submitted job is finished; all results."""
+        out: list[JobResult] = []
+        while self.tick(turns_per_tick) > 0:
+            out += self.poll()
+        return out + self.poll()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().hm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # the reference's entry points
 # ---------------------------------------------------------------------------

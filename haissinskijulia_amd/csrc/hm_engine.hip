@@ -281,7 +281,7 @@ hm_status Batch::build_tables() {
                 }
             }
             Fmax_ = std::max(Fmax_, q.d.F);
-            if (!(q.d.flags & F_GHIST)) pass_lds_ = std::max(pass_lds_, (size_t)q.d.nbins * 4);
+            pass_lds_ = std::max(pass_lds_, pass_hist_bytes(q.d) + (size_t)q.d.win * 8);
         }
     }
     ntiles_ = (int)tiles.size();
@@ -491,6 +491,10 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         d.nJ = (int32_t)((nw + Bk - 1) / Bk);
         if ((size_t)nb * 4 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
         if ((size_t)nb * 8 <= (size_t)ctx_->lds_max - 1024) d.flags |= F_PLDS;
+        {   // the pass's LDS window of P: what the histogram leaves of the LDS
+            const int64_t cap = ((int64_t)ctx_->lds_max - 1024 - (int64_t)pass_hist_bytes(d)) / 8;
+            d.win = cap >= 2 ? (int32_t)std::min<int64_t>(nb, cap) : 0;
+        }
         Work &w = q.b->w;
         HM_TRY(w.hist.ensure((size_t)nb * 2));
         HM_TRY(w.P.ensure((size_t)nb));
@@ -554,7 +558,8 @@ hm_status Batch::one_turn() {
         ev_used_ += 3;
         HIP_TRY(hipEventRecord(e0, s));
     }
-    HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, ntiles_, pass_lds_, std_only_ ? 1 : 0, s));
+    HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, ntiles_, std::min(ntiles_, ctx_->cus), pass_lds_,
+                             std_only_ ? 1 : 0, s));
     if (timing_) HIP_TRY(hipEventRecord(e1, s));
     if (any_wake_) {
         HIP_TRY(launch_conv(CONV_FUSED, d_bd_.p, d_conv_[CONV_FUSED].p, nconv_[CONV_FUSED], Fmax_, s));

@@ -352,21 +352,15 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, uint32_t *hist, do
 // One tile of the fused pass.  FX = kStdFlags is the common configuration
 // with the flags folded at compile time; FX = 0 handles any other.  Both
 // evaluate the same expressions, so a bunch gets the same bits from either.
+// `hist` is the LDS or global histogram, sP the LDS window of P.
 template <uint32_t FX>
-__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32_t *s_hist,
-                                               double *s_red) {
+__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32_t *hist,
+                                               const double *sP, double *s_red) {
     PassCtx c;
     load_ctx(B, c);
     const Flags<FX> fl{c.flags};
     const bool wake = fl.has(F_WAKE);
-    const bool lds_hist = wake && !fl.has(F_GHIST);
     const TurnScalars &sc = *B.sc;
-    uint32_t *ghist = hist_of(B, sc.turn);
-    uint32_t *hist = lds_hist ? s_hist : ghist;
-    if (lds_hist) {
-        for (int b = threadIdx.x; b < c.nb; b += blockDim.x) s_hist[b] = 0u;
-        __syncthreads();
-    }
     const int64_t n = B.n;
     const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
     const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
@@ -378,6 +372,8 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
     const double *nrm = inj ? B.normals + sc.turn * n : nullptr;
     const uint64_t seed = B.seed;
     const bool kick2 = c.pending_kick && c.nb >= 2;
+    const int wlo = sc.win_lo;
+    const unsigned wpairs = (unsigned)max(B.win - 1, 0);   // pair starts inside the window
     const double2 *__restrict__ P2 = reinterpret_cast<const double2 *>(B.P2);
     double *__restrict__ zp = B.z;
     double *__restrict__ ep = B.dE;
@@ -396,16 +392,19 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
             zn = *reinterpret_cast<const double2 *>(zp + 2 * pn);
             en = *reinterpret_cast<const double2 *>(ep + 2 * pn);
         }
-        // gathers of the previous turn's potential first, the RNG work hides
-        // them; (P[k], P[k+1]) come as one 16-byte load
+        // the previous turn's potential at (P[k], P[k+1]): from the LDS
+        // window, or one 16-byte global gather outside it
         int k0 = 0, k1 = 0;
         double w0 = 0.0, w1 = 0.0;
         double2 q0 = make_double2(0.0, 0.0), q1 = make_double2(0.0, 0.0);
         if (kick2) {
             interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
             interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
-            q0 = P2[k0];
-            q1 = P2[k1];
+            const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
+            if (j0 < wpairs) q0 = make_double2(sP[j0], sP[j0 + 1]);
+            else q0 = P2[k0];
+            if (j1 < wpairs) q1 = make_double2(sP[j1], sP[j1 + 1]);
+            else q1 = P2[k1];
         }
         double r0 = 0.0, r1 = 0.0;
         if (qe) {
@@ -464,36 +463,76 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
         double *o = B.part + 4 * (int64_t)lt;
         o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
     }
-    if (lds_hist) {
-        __syncthreads();
-        for (int b = threadIdx.x; b < c.nb; b += blockDim.x) {
-            const uint32_t h = s_hist[b];
-            if (h) atomicAdd(&ghist[b], h);
-        }
+    __syncthreads();   // s_red is reused by the next tile
+}
+
+// Adds the LDS histogram of bunch B into this turn's global counts.
+__device__ inline void flush_hist(const BunchDev &B, const uint32_t *s_hist) {
+    if (!pass_hist_bytes(B)) return;
+    __syncthreads();
+    uint32_t *g = hist_of(B, B.sc->turn);
+    for (int b = threadIdx.x; b < B.nbins; b += blockDim.x) {
+        const uint32_t h = s_hist[b];
+        if (h) atomicAdd(&g[b], h);
     }
 }
 
-// The fused pass over a tile table: k_turn_pass_std when every tile's bunch
-// has exactly the standard flags (kStdFlags, LDS histogram), else the
-// generic k_turn_pass.
+// Clears the LDS histogram of bunch B and loads its potential window.
+__device__ inline void stage_bunch(const BunchDev &B, uint8_t *lds) {
+    __syncthreads();   // the previous bunch's LDS is no longer read
+    const int hb = pass_hist_bytes(B);
+    uint32_t *h = reinterpret_cast<uint32_t *>(lds);
+    for (int b = threadIdx.x; b < hb / 4; b += blockDim.x) h[b] = 0u;
+    const TurnScalars &sc = *B.sc;
+    if (sc.pending_kick && B.win > 0) {
+        double *sP = reinterpret_cast<double *>(lds + hb);
+        const double *P = B.P + sc.win_lo;
+        for (int k = threadIdx.x; k < B.win; k += blockDim.x) sP[k] = P[k];
+    }
+    __syncthreads();
+}
+
+// The fused pass over a tile table.  Each workgroup takes a contiguous range
+// of tiles (one workgroup per CU), stages a bunch's LDS once for all of its
+// tiles in the range and flushes the histogram when it moves on.  The
+// per-tile sums depend on the tile and the block size only.
 template <uint32_t FX>
-__device__ __forceinline__ void turn_pass_entry(const BunchDev *__restrict__ bds,
-                                                const int32_t *__restrict__ tile_bunch) {
-    extern __shared__ uint32_t s_hist[];
+__device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bds,
+                                                 const int32_t *__restrict__ tile_bunch,
+                                                 int ntiles) {
+    extern __shared__ __align__(16) uint8_t s_lds[];
     __shared__ double s_red[4 * 16];
-    const int tile = blockIdx.x;
-    const BunchDev &B = bds[tile_bunch[tile]];
-    turn_pass_tile<FX>(B, tile - B.tile_begin, s_hist, s_red);
+    const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(ntiles, t0 + per);
+    uint32_t *s_hist = reinterpret_cast<uint32_t *>(s_lds);
+    int staged = -1;
+    for (int tile = t0; tile < t1; ++tile) {
+        const int bi = tile_bunch[tile];
+        const BunchDev &B = bds[bi];
+        if (bi != staged) {
+            if (staged >= 0) flush_hist(bds[staged], s_hist);
+            stage_bunch(B, s_lds);
+            staged = bi;
+        }
+        const int hb = pass_hist_bytes(B);
+        uint32_t *hist = hb ? s_hist : hist_of(B, B.sc->turn);
+        turn_pass_tile<FX>(B, tile - B.tile_begin, hist,
+                           reinterpret_cast<const double *>(s_lds + hb), s_red);
+    }
+    if (staged >= 0) flush_hist(bds[staged], s_hist);
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__restrict__ bds,
-                                                            const int32_t *__restrict__ tile_bunch) {
-    turn_pass_entry<0>(bds, tile_bunch);
+                                                            const int32_t *__restrict__ tile_bunch,
+                                                            int ntiles) {
+    turn_pass_blocks<0>(bds, tile_bunch, ntiles);
 }
 
+// the same for tables whose bunches all have exactly kStdFlags
 __global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
-    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch) {
-    turn_pass_entry<kStdFlags>(bds, tile_bunch);
+    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch, int ntiles) {
+    turn_pass_blocks<kStdFlags>(bds, tile_bunch, ntiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -781,19 +820,26 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_sum_lds(
         const double *__restrict__ zp = B.z;
         double v[1] = {0.0};
         if (nb >= 2) {
-            int64_t p = p0 + threadIdx.x;
-            double2 zn = make_double2(0.0, 0.0);
-            if (p < pf) zn = *reinterpret_cast<const double2 *>(zp + 2 * p);
-            for (; p < pf; p += blockDim.x) {
-                const double2 zz = zn;
-                if (p + blockDim.x < pf)
-                    zn = *reinterpret_cast<const double2 *>(zp + 2 * (p + blockDim.x));
-                int k0, k1;
-                double w0, w1;
-                interp_index(zz.x, c0, c1, inv, nb, k0, w0);
-                interp_index(zz.y, c0, c1, inv, nb, k1, w1);
-                v[0] += sP[k0] * (1.0 - w0) + sP[k0 + 1] * w0;
-                v[0] += sP[k1] * (1.0 - w1) + sP[k1 + 1] * w1;
+            // four pairs per thread in flight; each thread still adds its
+            // pairs in ascending order
+            const int64_t bs = blockDim.x;
+            for (int64_t p = p0 + threadIdx.x; p < pf; p += 4 * bs) {
+                double2 zz[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    zz[u] = (p + u * bs < pf)
+                                ? *reinterpret_cast<const double2 *>(zp + 2 * (p + u * bs))
+                                : make_double2(0.0, 0.0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (p + u * bs >= pf) break;
+                    int k0, k1;
+                    double w0, w1;
+                    interp_index(zz[u].x, c0, c1, inv, nb, k0, w0);
+                    interp_index(zz[u].y, c0, c1, inv, nb, k1, w1);
+                    v[0] += sP[k0] * (1.0 - w0) + sP[k0 + 1] * w0;
+                    v[0] += sP[k1] * (1.0 - w1) + sP[k1 + 1] * w1;
+                }
             }
         } else {
             for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) v[0] += sP[0] + sP[0];
@@ -883,26 +929,37 @@ __global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restri
         s.pending_m = 0;
         s.drift_skipped_count = 0;
         s.turn = 0;
+        s.win_lo = 0;
         *B.sc = s;
     }
 }
 
 __global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__restrict__ bds,
                                                             const int32_t *__restrict__ list) {
-    __shared__ double s_red[2 * 16];
+    __shared__ double s_red[3 * 16];
     const BunchDev &B = bds[list[blockIdx.x]];
     const uint32_t fl = B.flags;
-    double v[2] = {0.0, 0.0};
-    if (fl & F_E0) {
-        for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
+    double v[3] = {0.0, 0.0, 0.0};   // sum (dE'-dE_init), sum interpolated P, sum z
+    for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
+        if (fl & F_E0) {
             v[0] += B.part[4 * t + 2];
             if (fl & F_WAKE) v[1] += B.ipart[t];
         }
+        v[2] += B.part[4 * t];
     }
-    block_sum<2>(v, s_red);
+    block_sum<3>(v, s_red);
     if (threadIdx.x == 0) {
         TurnScalars s = *B.sc;
         double E0 = s.E0;
+        // the next pass's LDS window of P, centred on this turn's centroid
+        // (where the window sits changes no result, only where P is read)
+        s.win_lo = 0;
+        if ((fl & F_WAKE) && B.win > 0 && B.win < B.nbins && B.n > 0) {
+            const double kc = (v[2] / (double)B.n - B.c_first) * B.inv_s;
+            const double lo = floor(kc) - (double)(B.win / 2);
+            const double hi = (double)(B.nbins - B.win);
+            s.win_lo = (lo == lo) ? (int32_t)(lo < 0.0 ? 0.0 : (lo > hi ? hi : lo)) : 0;
+        }
         if (s.drift_skip) s.drift_skipped_count += 1;
         if (fl & F_E0) {                                  // evolution.jl:419-429
             const double mean = (v[0] - v[1]) / (double)B.n;
@@ -1081,13 +1138,15 @@ hipError_t launch_init(const BunchDev *bds, const int32_t *list, int nb, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, int ntiles, size_t lds,
-                            int std_only, hipStream_t s) {
-    if (ntiles <= 0) return hipSuccess;
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, int ntiles, int grid,
+                            size_t lds, int std_only, hipStream_t s) {
+    if (ntiles <= 0 || grid <= 0) return hipSuccess;
     if (std_only)
-        hipLaunchKernelGGL(k_turn_pass_std, dim3(ntiles), dim3(kPassThreads), lds, s, bds, tb);
+        hipLaunchKernelGGL(k_turn_pass_std, dim3((unsigned)grid), dim3(kPassThreads), lds, s, bds,
+                           tb, ntiles);
     else
-        hipLaunchKernelGGL(k_turn_pass, dim3(ntiles), dim3(kPassThreads), lds, s, bds, tb);
+        hipLaunchKernelGGL(k_turn_pass, dim3((unsigned)grid), dim3(kPassThreads), lds, s, bds, tb,
+                           ntiles);
     return hipGetLastError();
 }
 

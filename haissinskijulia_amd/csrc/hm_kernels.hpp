@@ -9,7 +9,7 @@
 
 namespace hm {
 
-constexpr int kPassThreads = 512;      // turn-pass / moments / interp-sum block
+constexpr int kPassThreads = 1024;     // turn-pass / moments / interp-sum block
 constexpr int kConvThreads = 256;      // convolution block
 constexpr int kConvInvThreads = 1024;  // phase-B convolution block (spectral sums in flight)
 constexpr int kFinalThreads = 256;     // per-bunch finalize block
@@ -30,8 +30,10 @@ hipError_t launch_moments(const BunchDev *bds, const int32_t *tile_bunch, int nt
 hipError_t launch_moments_final(const BunchDev *bds, const int32_t *bunch_list, int nb, int mode,
                                 hipStream_t s);
 hipError_t launch_init(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
-// std_only: every tile's bunch has (flags & kPhysMask) == kStdFlags
-hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
+// `grid` workgroups over the tile table; lds_bytes = the largest per-bunch
+// histogram + potential window; std_only: every tile's bunch has
+// (flags & kPhysMask) == kStdFlags
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int grid,
                             size_t lds_bytes, int std_only, hipStream_t s);
 // items are (bunch, block) pairs; Fmax is the largest FFT length among them
 hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int nitems, int Fmax,

@@ -15,6 +15,7 @@
 //   sc                          TurnScalars of the current turn
 //   out[8]                      sigma_E0, sigma_z0, sigma_E, sigma_z, E0, ...
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace hm {
@@ -58,7 +59,8 @@ struct TurnScalars {
     int32_t drift_skip;     // apply_phase_advance! skip condition of this turn
     int32_t drift_skipped_count;
     int64_t turn;           // turns completed in this call
-    int64_t pad_;
+    int32_t win_lo;         // first bin of the pass's LDS window of P (around the centroid)
+    int32_t pad_;
 };
 
 struct BunchDev {
@@ -87,9 +89,17 @@ struct BunchDev {
     const double *What;   // interleaved complex [nJ][F][2]
     const double *tw;     // interleaved complex [F/2][2], exp(-2 pi i k / F)
     double *X;            // interleaved complex [nA][F][2]: FFT(g_a + i g_{a-1}) (nJ > 1)
-    int32_t B, F, logF, nA, nJ, pad_;
+    int32_t B, F, logF, nA, nJ;
+    int32_t win;          // entries of P the pass stages in LDS (0: all gathers global)
     double *part, *ipart, *moments, *out;
     TurnScalars *sc;
 };
+
+// LDS of a turn-pass workgroup for one bunch: its histogram (nbins counts,
+// unless F_GHIST), then a window of `win` entries of the previous turn's
+// potential, P[win_lo .. win_lo+win), for the wake-kick gathers.
+__host__ __device__ inline int pass_hist_bytes(const BunchDev &B) {
+    return ((B.flags & F_WAKE) && !(B.flags & F_GHIST)) ? ((B.nbins * 4 + 15) & ~15) : 0;
+}
 
 }  // namespace hm

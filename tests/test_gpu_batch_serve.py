@@ -1,0 +1,173 @@
+"""Batched tracking (many bunches per launch) and the serving loop.
+
+A bunch's tiling, reduction order and random stream depend on that bunch
+alone, so running it inside a batch or a served mix must give the same bits
+as running it alone; these tests hold the batched and served paths to that
+(bitwise), plus cancel, the failure hook and HBM-budget admission.
+"""
+import numpy as np
+import pytest
+
+from helpers import COORD_RTOL, W, normwise
+from oracle import oracle as O
+
+import haissinskijulia_amd as H
+from haissinskijulia_amd._lib import HM_ERR_CANCELLED as CANCELLED
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(turns, V=3.2e6, flags=(True, True, True, True, True)):
+    return H.SimulationParameters.from_dict(W.ring_3gev_lattice(turns, voltage=V, flags=flags))
+
+
+def _specs():
+    """Uneven bunches: sizes, bins, voltages, charges, flags and seeds differ."""
+    return [
+        dict(n=50_001, nbins=1024, V=3.2e6, charge=1e-9, seed=11, flags=(True, True, True, True, True)),
+        dict(n=7_000, nbins=256, V=2.8e6, charge=4e-9, seed=12, flags=(True, True, True, True, True)),
+        dict(n=120_000, nbins=4096, V=3.9e6, charge=8e-9, seed=13, flags=(True, True, True, True, True)),
+        dict(n=3_333, nbins=512, V=3.0e6, charge=0.0, seed=14, flags=(True, False, True, True, True)),
+        dict(n=20_000, nbins=64, V=3.5e6, charge=0.0, seed=15, flags=(False, True, False, True, True)),
+    ]
+
+
+def _opts(s):
+    return H.TrackingOptions(seed=s["seed"] + 1000, charge=s["charge"],
+                             resonators=[W.RESONATOR_BB] if s["charge"] else (),
+                             bin_half_width_sigmas=8.0)
+
+
+def _bunch(s):
+    b = H.DeviceBunch(s["n"])
+    b.generate(s["seed"], W.SIGMA_Z, W.SIGMA_E)
+    return b
+
+
+@pytest.mark.parametrize("turns", [1, 40])
+def test_batch_equals_each_bunch_alone_bitwise(turns):
+    specs = _specs()
+    alone = []
+    for s in specs:
+        b = _bunch(s)
+        r = b.evolve(_params(turns, s["V"], s["flags"]), s["nbins"], _opts(s))
+        alone.append((r, *b.download()))
+        b.close()
+    bunches = [_bunch(s) for s in specs]
+    rs = H.batch_evolve(bunches, [s["nbins"] for s in specs],
+                        [_params(turns, s["V"], s["flags"]) for s in specs],
+                        [_opts(s) for s in specs])
+    for s, b, r, (ra, za, ea) in zip(specs, bunches, rs, alone):
+        z, e = b.download()
+        print(f"n={s['n']} nbins={s['nbins']} sigma_E={r.sigma_E:.6e} E0={r.E0:.6e}", flush=True)
+        assert np.array_equal(z, za) and np.array_equal(e, ea)
+        assert (r.sigma_E, r.sigma_z, r.E0, r.turns_done) == \
+               (ra.sigma_E, ra.sigma_z, ra.E0, ra.turns_done)
+        b.close()
+
+
+def test_batch_of_uneven_turn_counts_matches_alone():
+    """Bunches leave the batch at different turns; the others continue."""
+    specs = _specs()[:3]
+    turns = [5, 17, 11]
+    alone = []
+    for s, t in zip(specs, turns):
+        b = _bunch(s)
+        r = b.evolve(_params(t, s["V"]), s["nbins"], _opts(s))
+        alone.append((r, *b.download()))
+        b.close()
+    bunches = [_bunch(s) for s in specs]
+    rs = H.batch_evolve(bunches, [s["nbins"] for s in specs],
+                        [_params(t, s["V"]) for s, t in zip(specs, turns)],
+                        [_opts(s) for s in specs])
+    for b, r, t, (ra, za, ea) in zip(bunches, rs, turns, alone):
+        z, e = b.download()
+        assert r.turns_done == t
+        assert np.array_equal(z, za) and np.array_equal(e, ea)
+        assert r.E0 == ra.E0
+        b.close()
+
+
+def test_batched_bunch_matches_oracle():
+    """One bunch of a batch against the oracle (injected initial state and
+    the device's own Philox stream restated by the oracle)."""
+    s = _specs()[0]
+    turns = 20
+    p = W.ring_3gev_lattice(turns, voltage=s["V"])
+    z0, e0 = O.generate_gaussian(s["seed"], 0.0, W.SIGMA_Z, 0.0, W.SIGMA_E, s["n"])
+    zr, er, rr = O.evolve(z0, e0, s["nbins"], p, seed=s["seed"] + 1000, charge=s["charge"],
+                          resonators=[W.RESONATOR_BB], bin_half_width_sigmas=8.0)
+    assert rr.status == 0
+    target = H.DeviceBunch(s["n"])
+    target.upload(z0, e0)
+    others = [_bunch(x) for x in _specs()[1:3]]
+    bunches = [others[0], target, others[1]]
+    specs = [_specs()[1], s, _specs()[2]]
+    rs = H.batch_evolve(bunches, [x["nbins"] for x in specs],
+                        [_params(turns, x["V"]) for x in specs], [_opts(x) for x in specs])
+    z, e = target.download()
+    ez, ee = normwise(z, zr), normwise(e, er)
+    print(f"normwise z={ez:.2e} dE={ee:.2e}", flush=True)
+    assert ez <= COORD_RTOL and ee <= COORD_RTOL
+    assert rs[1].E0 == pytest.approx(rr.E0, rel=1e-14)
+    for b in bunches:
+        b.close()
+
+
+def _job(s, turns, **kw):
+    return H.Job(params=_params(turns, s["V"], s["flags"]), n=s["n"], nbins=s["nbins"],
+                 particle_seed=s["seed"], sigma_z=W.SIGMA_Z, sigma_E=W.SIGMA_E, options=_opts(s), **kw)
+
+
+def test_served_jobs_equal_alone_runs_and_finish_in_order_of_work():
+    specs = _specs()
+    turns = [30, 12, 25, 7, 18]
+    srv = H.Server(budget_bytes=8 << 30, max_slots=8)
+    ids = [srv.submit(_job(s, t)) for s, t in zip(specs, turns)]
+    res = {r.job_id: r for r in srv.run(turns_per_tick=5)}
+    assert sorted(res) == sorted(ids)
+    for s, t, jid in zip(specs, turns, ids):
+        b = _bunch(s)
+        ra = b.evolve(_params(t, s["V"], s["flags"]), s["nbins"], _opts(s))
+        z, e = b.download()
+        b.close()
+        r = res[jid]
+        assert r.status == 0
+        assert r.result.turns_done == t
+        assert (r.result.sigma_E, r.result.sigma_z, r.result.E0) == (ra.sigma_E, ra.sigma_z, ra.E0)
+        assert r.centroid_z == pytest.approx(float(np.mean(z)), rel=1e-9, abs=1e-15)
+        assert r.t_submit <= r.t_admit <= r.t_done
+    srv.close()
+
+
+def test_cancel_and_failure_hook():
+    specs = _specs()
+    srv = H.Server(budget_bytes=8 << 30, max_slots=2)
+    a = srv.submit(_job(specs[0], 40))
+    b = srv.submit(_job(specs[1], 40, fail_after_turns=10))
+    c = srv.submit(_job(specs[2], 40))     # queued behind the two slots
+    d = srv.submit(_job(specs[3], 15))
+    srv.tick(5)
+    srv.cancel(a)                          # resident
+    srv.cancel(c)                          # may be queued or resident by now
+    res = {r.job_id: r for r in srv.run(turns_per_tick=5)}
+    assert res[a].status == CANCELLED and res[a].result.turns_done < 40
+    assert res[b].status == CANCELLED and res[b].result.turns_done == 10
+    assert res[c].status == CANCELLED
+    assert res[d].status == 0 and res[d].result.turns_done == 15
+    srv.close()
+
+
+def test_budget_admits_one_job_at_a_time():
+    specs = _specs()[:3]
+    # the jobs need about 1.2, 0.5 and 2.6 MB (particles, bin tables, FFT
+    # work); 1.5 MB holds no two of them, and the 2.6 MB one runs alone
+    srv = H.Server(budget_bytes=1_500_000, max_slots=8)
+    ids = [srv.submit(_job(s, 6)) for s in specs]
+    res = [r for r in srv.run(turns_per_tick=3)]
+    order = {r.job_id: r for r in res}
+    adm = sorted((order[i].t_admit, order[i].t_done) for i in ids)
+    for (a0, d0), (a1, _) in zip(adm, adm[1:]):
+        assert a1 >= d0 - 1e-9          # the next job starts after the previous one ends
+    assert all(order[i].status == 0 for i in ids)
+    srv.close()
