@@ -65,7 +65,7 @@ typedef struct {
     int32_t timing;             /* 1: time the turn-pass kernel with HIP events */
     const double *resonators;   /* host [n_resonators][3] = (R_s [Ohm], f_r [Hz], Q) */
     int32_t fft_block;          /* 0 = automatic; else the convolution block size (power of 2) */
-    int32_t block_size;         /* 0 = automatic; else threads per block of the turn pass */
+    int32_t block_size;         /* 0 = automatic; else threads per block of the turn pass (1024) */
 } hm_options;
 
 /* Return of longitudinal_evolve! (evolution.jl:470) plus diagnostics. */
