@@ -39,8 +39,21 @@ def log(msg: str) -> None:
 def tracking_options(wl: dict, seed: int, timing: bool = False):
     import haissinskijulia_amd as H
     return H.TrackingOptions(seed=seed, resonators=wl["resonators"], charge=wl["charge"],
-                             bin_half_width_sigmas=wl["half_width"], delta_sigma=-1.0,
-                             timing=timing)
+                             bin_half_width_sigmas=wl["half_width"], timing=timing)
+
+
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None if there is none)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def cpu_baseline(wl: dict, n: int, turns: int) -> dict:
@@ -52,13 +65,128 @@ def cpu_baseline(wl: dict, n: int, turns: int) -> dict:
     t0 = time.perf_counter()
     _, _, r = O.evolve(z, e, wl["nbins"], p, seed=wl["seed"], nranks=cores,
                        resonators=wl["resonators"], charge=wl["charge"],
-                       bin_half_width_sigmas=wl["half_width"], delta_sigma=-1.0)
+                       bin_half_width_sigmas=wl["half_width"])
     dt = time.perf_counter() - t0
     assert r.status == 0
     return {"value": n * turns / dt, "unit": "particle-turns/s", "cores": cores, "kind": "port",
             "sample": f"{n:.0e} particles x {turns} turns of the same lattice, bins and wake; "
                       f"oracle C restatement of the reference loop (MPI branch), "
                       f"{cores} OpenMP rank-equivalents, {dt:.1f} s"}
+
+
+def _pct(x, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(x, dtype=float), q)) if len(x) else None
+
+
+def bench_scan(dev, turns: int) -> dict:
+    """256 uneven bunches (a parameter scan) advanced together by
+    hm_batch_evolve, against the plain path: the same bunches evolved one
+    after another.  All bunches are submitted at t = 0, so a bunch's latency
+    is its completion time."""
+    import haissinskijulia_amd as H
+    from haissinskijulia_amd import workloads as W
+    specs = W.scan_bunches()
+
+    def params(s):
+        return H.SimulationParameters.from_dict(W.ring_3gev_lattice(turns, voltage=s["V"]))
+
+    def opts(s):
+        return H.TrackingOptions(seed=s["seed"] + 1, resonators=s["resonators"], charge=s["charge"],
+                                 bin_half_width_sigmas=s["half_width"])
+
+    plain, done = [], 0.0
+    for s in specs:
+        b = H.DeviceBunch(s["n"], dev)
+        b.generate(s["seed"], W.SIGMA_Z, W.SIGMA_E)
+        t0 = time.perf_counter()
+        r = b.evolve(params(s), s["nbins"], opts(s))
+        done += time.perf_counter() - t0
+        plain.append((done, r))
+        b.close()
+    bunches = []
+    for s in specs:
+        b = H.DeviceBunch(s["n"], dev)
+        b.generate(s["seed"], W.SIGMA_Z, W.SIGMA_E)
+        bunches.append(b)
+    t0 = time.perf_counter()
+    rs = H.batch_evolve(bunches, [s["nbins"] for s in specs], [params(s) for s in specs],
+                        [opts(s) for s in specs])
+    t_batch = time.perf_counter() - t0
+    for b in bunches:
+        b.close()
+    same = all((a.sigma_E, a.sigma_z, a.E0) == (b.sigma_E, b.sigma_z, b.E0)
+               for a, (_, b) in zip(rs, plain))
+    work = sum(s["n"] for s in specs) * turns
+    lat_plain = [t for t, _ in plain]
+    return {"workload": f"{len(specs)} bunches, N log-uniform [1e4, 4e6] (sum {work // turns:.3g}), "
+                        f"V U[2.5, 4.0] MV, q U[0.5, 10] nC, 4096 bins, {turns} turns, "
+                        "seeds from SeedSequence(3)",
+            "batched_s": t_batch, "plain_s": lat_plain[-1], "speedup": lat_plain[-1] / t_batch,
+            "batched_particle_turns_per_s": work / t_batch,
+            "plain_particle_turns_per_s": work / lat_plain[-1],
+            "p50_s": t_batch, "p99_s": t_batch,
+            "plain_p50_s": _pct(lat_plain, 50), "plain_p99_s": _pct(lat_plain, 99),
+            "results_identical_to_plain": same}
+
+
+def bench_serve(dev, count: int, rate: float, turns_per_tick: int) -> dict:
+    """Continuous batching (hm_server) of a Poisson job stream in real time,
+    against the plain path serving the same stream one job at a time (FIFO;
+    its latencies follow from the measured service times and the arrivals)."""
+    import haissinskijulia_amd as H
+    from haissinskijulia_amd import workloads as W
+    jobs = W.serve_jobs(count, rate, n_range=(1e4, 1e6), turn_range=(100, 2000))
+
+    def job(j):
+        return H.Job(params=H.SimulationParameters.from_dict(
+            W.ring_3gev_lattice(j["turns"], voltage=j["V"])), n=j["n"], nbins=j["nbins"],
+            particle_seed=j["seed"], sigma_z=W.SIGMA_Z, sigma_E=W.SIGMA_E,
+            options=H.TrackingOptions(seed=j["seed"] + 1, resonators=j["resonators"],
+                                      charge=j["charge"], bin_half_width_sigmas=j["half_width"]))
+
+    # plain path: service time of each job alone
+    service = []
+    for j in jobs:
+        b = H.DeviceBunch(j["n"], dev)
+        t0 = time.perf_counter()
+        b.generate(j["seed"], W.SIGMA_Z, W.SIGMA_E)
+        spec = job(j)
+        b.evolve(spec.params, spec.nbins, spec.options)
+        service.append(time.perf_counter() - t0)
+        b.close()
+    lat_plain, free = [], 0.0
+    for j, s in zip(jobs, service):
+        free = max(free, j["arrival"]) + s
+        lat_plain.append(free - j["arrival"])
+    # served, in real time
+    srv = H.Server(budget_bytes=200 << 30, max_slots=64, device=dev)
+    t0 = time.perf_counter()
+    submitted, lat, k = {}, [], 0
+    while len(lat) < len(jobs):
+        now = time.perf_counter() - t0
+        while k < len(jobs) and jobs[k]["arrival"] <= now:
+            submitted[srv.submit(job(jobs[k]))] = jobs[k]["arrival"]
+            k += 1
+        active = srv.tick(turns_per_tick)
+        tnow = time.perf_counter() - t0
+        for r in srv.poll():
+            assert r.status == 0, r.status
+            lat.append(tnow - submitted[r.job_id])
+        if active == 0 and k < len(jobs):
+            time.sleep(max(0.0, jobs[k]["arrival"] - (time.perf_counter() - t0)))
+    span = time.perf_counter() - t0
+    srv.close()
+    work = sum(j["n"] * j["turns"] for j in jobs)
+    span_plain = max(free, 0.0)
+    return {"workload": f"{count} jobs, Poisson {rate:g}/s, N log-uniform [1e4, 1e6], turns "
+                        f"log-uniform [100, 2000], 20% FD pairs (V x (1 -+ 0.5%), common random "
+                        f"numbers), 1024 bins, {turns_per_tick} turns per tick",
+            "p50_s": _pct(lat, 50), "p99_s": _pct(lat, 99),
+            "particle_turns_per_s": work / span,
+            "plain_p50_s": _pct(lat_plain, 50), "plain_p99_s": _pct(lat_plain, 99),
+            "plain_particle_turns_per_s": work / span_plain,
+            "offered_particle_turns_per_s": work / jobs[-1]["arrival"]}
 
 
 def main() -> int:
@@ -72,6 +200,12 @@ def main() -> int:
     ap.add_argument("--cpu-particles", type=int, default=4_000_000)
     ap.add_argument("--cpu-turns", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scan-turns", type=int, default=200,
+                    help="turns of the 256-bunch scan (batched vs plain); 0 skips it")
+    ap.add_argument("--serve-jobs", type=int, default=300,
+                    help="jobs of the serving run (server vs plain FIFO); 0 skips it")
+    ap.add_argument("--serve-rate", type=float, default=150.0, help="job arrivals per second")
+    ap.add_argument("--serve-tick", type=int, default=25, help="turns per server tick")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -116,6 +250,7 @@ def main() -> int:
     bunch.close()
     copy_ms, copy_gbs = H.api.stream_copy(int(16 * n), 10, dev)
     log(f"plain copy of {16 * n / 1e9:.1f} GB: {copy_ms:.3f} ms -> {copy_gbs:.0f} GB/s")
+    traffic, traffic_src = profiled_traffic("k_turn_pass", args.workload)
 
     out = {
         "metric": METRIC,
@@ -137,13 +272,25 @@ def main() -> int:
                    "flags": "wakefield, update_eta, update_E0, SR_damping, excitation",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_turn_pass", "kernel_ms": pass_ms,
                      "algorithmic_bytes_per_launch": BYTES_PER_PARTICLE_TURN * n},
         "whole_loop_frac_of_hbm_peak": value * BYTES_PER_PARTICLE_TURN / 1e9 / HBM_PEAK_GBS,
         "stream_copy_GBps": copy_gbs,
         "conv_ms": rt.ms_conv_avg,
     }
+    if args.scan_turns > 0:
+        log(f"scan: 256 bunches x {args.scan_turns} turns, plain then batched ...")
+        out["scan"] = bench_scan(dev, args.scan_turns)
+        log(f"scan: batched {out['scan']['batched_s']:.2f} s, plain {out['scan']['plain_s']:.2f} s "
+            f"-> {out['scan']['speedup']:.1f}x, identical={out['scan']['results_identical_to_plain']}")
+    if args.serve_jobs > 0:
+        log(f"serving: {args.serve_jobs} jobs at {args.serve_rate:g}/s, plain then served ...")
+        out["serving"] = bench_serve(dev, args.serve_jobs, args.serve_rate, args.serve_tick)
+        sv = out["serving"]
+        log(f"serving: p50 {sv['p50_s']:.3f} s p99 {sv['p99_s']:.3f} s; plain p50 "
+            f"{sv['plain_p50_s']:.3f} s p99 {sv['plain_p99_s']:.3f} s")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_particles, args.cpu_turns)

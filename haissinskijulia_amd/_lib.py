@@ -112,6 +112,9 @@ def load() -> C.CDLL:
             "hm_bunch_upload": (st, [vp, _dp, _dp]),
             "hm_bunch_download": (st, [vp, _dp, _dp]),
             "hm_bunch_generate": (st, [vp, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double]),
+            "hm_bunch_generate_particles": (st, [vp, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                                 C.c_double, C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double)]),
             "hm_bunch_evolve": (st, [vp, C.c_int64, C.POINTER(HmParams), C.POINTER(HmOptions),
                                      C.POINTER(HmResult)]),
             "hm_bunch_device_ptrs": (st, [vp, C.POINTER(vp), C.POINTER(vp)]),

@@ -295,7 +295,17 @@ class DeviceBunch:
 
     def generate(self, seed: int, sigma_z: float, sigma_E: float, mu_z: float = 0.0,
                  mu_E: float = 0.0) -> None:
+        """Independent Gaussian z and dE from the Philox stream of `seed`."""
         check(_lib.load().hm_bunch_generate(self._h, seed, mu_z, sigma_z, mu_E, sigma_E))
+
+    def generate_particles(self, seed: int, mu_z: float, mu_E: float, sigma_z: float,
+                           sigma_E: float) -> tuple[float, float]:
+        """generate_particles (src/evolution.jl:42-133) into this bunch;
+        returns (σ_E, σ_z) as the reference does."""
+        sE, sz = C.c_double(), C.c_double()
+        check(_lib.load().hm_bunch_generate_particles(self._h, seed, mu_z, mu_E, sigma_z, sigma_E,
+                                                      C.byref(sE), C.byref(sz)))
+        return sE.value, sz.value
 
     def evolve(self, params: SimulationParameters, nbins: int,
                options: Optional[TrackingOptions] = None) -> EvolveResult:
@@ -609,22 +619,23 @@ def bin_grid(z_width: float, nbins: int):
 
 def generate_particles(μ_z, μ_E, σ_z, σ_E, num_particles, energy, mass, ϕs, freq_rf,
                        seed: int = 1234, comm=None):
-    """generate_particles (src/evolution.jl:42-133) for independent Gaussian
-    coordinates drawn on the GPU from the Philox stream of `seed`.  Returns
-    (particles, σ_E, σ_z, energy) with the spreads of the drawn sample
-    (compute_std, src/utils.jl:531-544)."""
+    """generate_particles (src/evolution.jl:42-133) on the GPU: the sample
+    covariance of min(N, 10000) independent draws defines a MvNormal from which
+    the N particles are drawn (N >= 2); N == 1 draws independent normals;
+    N <= 0 returns an empty bunch.  Returns (particles, σ_E, σ_z, energy) with
+    the bunch's compute_std spreads (src/utils.jl:531-544), or the inputs for
+    N == 1, or zeros for N <= 0.  Draws come from the Philox stream of `seed`.
+    ``mass``, ``ϕs`` and ``freq_rf`` are accepted and unused, as in the
+    reference."""
     if num_particles <= 0:
         return Particles(np.empty(0), np.empty(0)), 0.0, 0.0, energy
     b = DeviceBunch(num_particles, _dev(comm))
     try:
-        b.generate(seed, σ_z, σ_E, μ_z, μ_E)
+        sE, sz = b.generate_particles(seed, μ_z, μ_E, σ_z, σ_E)
         z, dE = b.download()
     finally:
         b.close()
-    p = Particles(z, dE)
-    if num_particles < 2:
-        return p, σ_E, σ_z, energy
-    return p, float(np.std(dE, ddof=1)), float(np.std(z, ddof=1)), energy
+    return Particles(z, dE), sE, sz, energy
 
 
 def z_to_ϕ(z_val, rf_factor, ϕs):

@@ -220,6 +220,7 @@ static int64_t pairs_per_tile_for(int64_t npairs, int threads) {
 }
 
 Batch::~Batch() {
+    drop_graph();
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     d_bd_.release(); d_tiles_.release(); d_list_.release();
     for (auto &c : d_conv_) c.release();
@@ -249,9 +250,8 @@ hm_status Batch::build_tables() {
     std::vector<int32_t> tiles, conv[3], list;
     pass_lds_ = 0;
     Fmax_ = 2;
-    any_wake_ = any_wake_e0_ = any_plds_ = false;
+    any_wake_ = any_wake_e0_ = false;
     std_only_ = true;
-    interp_lds_ = 0;
     for (size_t i = 0; i < rb_.size(); ++i) {
         RunBunch &q = rb_[i];
         if (!used_[i] || !q.active) continue;
@@ -261,14 +261,7 @@ hm_status Batch::build_tables() {
         if ((q.d.flags & kPhysMask) != kStdFlags) std_only_ = false;
         if (q.d.flags & F_WAKE) {
             any_wake_ = true;
-            if (q.d.flags & F_E0) {
-                if (q.d.flags & F_PLDS) {
-                    any_plds_ = true;
-                    interp_lds_ = std::max(interp_lds_, (size_t)q.d.nbins * 8);
-                } else {
-                    any_wake_e0_ = true;
-                }
-            }
+            if (q.d.flags & F_E0) any_wake_e0_ = true;
             for (int o = 0; o < q.d.nA; ++o) {
                 if (q.d.nJ <= 1) {
                     conv[CONV_FUSED].push_back((int32_t)i);
@@ -490,7 +483,6 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         d.nA = (int32_t)((nb + Bk - 1) / Bk);
         d.nJ = (int32_t)((nw + Bk - 1) / Bk);
         if ((size_t)nb * 4 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
-        if ((size_t)nb * 8 <= (size_t)ctx_->lds_max - 1024) d.flags |= F_PLDS;
         {   // the pass's LDS window of P: what the histogram leaves of the LDS
             const int64_t cap = ((int64_t)ctx_->lds_max - 1024 - (int64_t)pass_hist_bytes(d)) / 8;
             d.win = cap >= 2 ? (int32_t)std::min<int64_t>(nb, cap) : 0;
@@ -567,10 +559,8 @@ hm_status Batch::one_turn() {
         HIP_TRY(launch_conv(CONV_INV, d_bd_.p, d_conv_[CONV_INV].p, nconv_[CONV_INV], Fmax_, s));
     }
     if (timing_) HIP_TRY(hipEventRecord(e2, s));
-    if (any_wake_e0_) HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, ntiles_, s));
-    if (any_plds_)
-        HIP_TRY(launch_interp_sum_lds(d_bd_.p, d_tiles_.p, ntiles_, std::min(ntiles_, ctx_->cus),
-                                      interp_lds_, s));
+    if (any_wake_e0_)   // two 64 KiB-LDS workgroups per CU
+        HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, ntiles_, std::min(ntiles_, 2 * ctx_->cus), s));
     HIP_TRY(launch_finalize(d_bd_.p, d_list_.p, nlist_, s));
     return HM_OK;
 }
@@ -593,8 +583,40 @@ hm_status Batch::flush_timing() {
     return HM_OK;
 }
 
+void Batch::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+}
+
+// Captures kGraphTurns turns of the current tables into one executable
+// graph (kernel nodes only: the turn reads all of its state from device
+// memory, so replays advance the bunches like direct launches).
+hm_status Batch::capture_graph() {
+    drop_graph();
+    hipStream_t s = ctx_->stream;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hm_status st = HM_OK;
+    for (int t = 0; t < kGraphTurns && st == HM_OK; ++t) st = one_turn();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st != HM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess)
+        return fail(HM_ERR_DEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        graph_exec_ = nullptr;
+        return fail(HM_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    return HM_OK;
+}
+
 hm_status Batch::advance(int64_t turns) {
-    for (int64_t step = 0;; ++step) {
+    int64_t step = 0;
+    for (;;) {
         std::vector<int32_t> retiring;
         for (size_t i = 0; i < rb_.size(); ++i)
             if (used_[i] && rb_[i].active && rb_[i].turns_done >= rb_[i].n_turns)
@@ -604,10 +626,25 @@ hm_status Batch::advance(int64_t turns) {
             HM_TRY(epilogue(retiring));
         }
         if (step >= turns || active_count() == 0) break;
-        if (dirty_) HM_TRY(build_tables());
-        HM_TRY(one_turn());
+        if (dirty_) {
+            drop_graph();
+            HM_TRY(build_tables());
+        }
+        // turns every active bunch still runs, within this call's budget
+        int64_t left = turns - step;
         for (size_t i = 0; i < rb_.size(); ++i)
-            if (used_[i] && rb_[i].active) rb_[i].turns_done++;
+            if (used_[i] && rb_[i].active) left = std::min(left, rb_[i].n_turns - rb_[i].turns_done);
+        int64_t did = 1;
+        if (!timing_ && left >= kGraphTurns) {
+            if (!graph_exec_) HM_TRY(capture_graph());
+            HIP_TRY(hipGraphLaunch(graph_exec_, ctx_->stream));
+            did = kGraphTurns;
+        } else {
+            HM_TRY(one_turn());
+        }
+        step += did;
+        for (size_t i = 0; i < rb_.size(); ++i)
+            if (used_[i] && rb_[i].active) rb_[i].turns_done += did;
     }
     HM_TRY(flush_timing());
     HIP_TRY(hipStreamSynchronize(ctx_->stream));
@@ -826,6 +863,37 @@ hm_status hm_bunch_generate(hm_bunch *b, uint64_t seed, double mu_z, double sigm
     HM_TRY(set_device(b->ctx));
     HIP_TRY(launch_generate(seed, mu_z, sigma_z, mu_E, sigma_E, b->z, b->dE, b->n, b->ctx->stream));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return HM_OK;
+}
+
+hm_status hm_bunch_generate_particles(hm_bunch *b, uint64_t seed, double mu_z, double mu_E,
+                                     double sigma_z, double sigma_E, double *sigma_E_out,
+                                     double *sigma_z_out) {
+    if (!b || !sigma_E_out || !sigma_z_out) return fail(HM_ERR_ARG, "bad arguments");
+    std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+    HM_TRY(set_device(b->ctx));
+    hipStream_t s = b->ctx->stream;
+    if (b->n <= 0) {                        // evolution.jl:47-53
+        *sigma_E_out = 0.0;
+        *sigma_z_out = 0.0;
+        return HM_OK;
+    }
+    if (b->n == 1) {                        // evolution.jl:61-66: the input spreads
+        HIP_TRY(launch_generate(seed, mu_z, sigma_z, mu_E, sigma_E, b->z, b->dE, 1, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        *sigma_E_out = sigma_E;
+        *sigma_z_out = sigma_z;
+        return HM_OK;
+    }
+    HM_TRY(b->w.moments.ensure((size_t)kGenWorkDoubles));
+    double *work = b->w.moments.p;
+    HIP_TRY(launch_generate_particles(seed, mu_z, mu_E, sigma_z, sigma_E, b->z, b->dE, b->n, work,
+                                      s));
+    double stats[4];
+    HIP_TRY(hipMemcpyAsync(stats, work + 4, sizeof(stats), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *sigma_z_out = stats[2];
+    *sigma_E_out = stats[3];
     return HM_OK;
 }
 

@@ -157,6 +157,11 @@ class Batch {
     hm_status epilogue(const std::vector<int32_t> &which);
     hm_status one_turn();
     hm_status flush_timing();
+    hm_status capture_graph();
+    void drop_graph();
+
+    static constexpr int kGraphTurns = 16;   // turns per captured graph
+    hipGraphExec_t graph_exec_ = nullptr;    // kGraphTurns turns of the current tables
 
     hm_ctx *ctx_;
     std::vector<RunBunch> rb_;
@@ -168,10 +173,8 @@ class Batch {
     int nconv_[3] = {0, 0, 0};
     size_t pass_lds_ = 0;
     int Fmax_ = 2;
-    bool any_wake_ = false, any_wake_e0_ = false;   // wake-E0 bunches with P in global memory
-    bool any_plds_ = false;                         // wake-E0 bunches with P staged in LDS
+    bool any_wake_ = false, any_wake_e0_ = false;   // some bunch with wake / wake + update_E0
     bool std_only_ = true;                          // all active bunches have kStdFlags
-    size_t interp_lds_ = 0;
     bool dirty_ = true;
     bool timing_ = false;
     std::vector<hipEvent_t> ev_pool_;

@@ -338,13 +338,45 @@ __device__ __forceinline__ void advance(const PassCtx &c, Flags<FX> fl, double &
     de_turn = e - e_init;
 }
 
-__device__ __forceinline__ void bin_and_sum(const PassCtx &c, uint32_t *hist, double z,
+// Explicit address spaces.  Pointers read from a BunchDev are generic to the
+// compiler, and a choice between LDS and global memory merges into one
+// generic access: both become flat instructions, which wait on the LDS and
+// the vector-memory counters together.  The pass therefore names global
+// memory (address space 1) and reaches its LDS through the kernel's own
+// dynamic-LDS symbol.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) double g_f64;
+typedef __attribute__((address_space(1))) f64x2 g_f64x2;
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+
+__device__ __forceinline__ double2 ld2(const g_f64x2 *a) {
+    const f64x2 v = *a;
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(g_f64x2 *a, double x, double y) {
+    f64x2 v;
+    v.x = x;
+    v.y = y;
+    *a = v;
+}
+
+__device__ __forceinline__ uint8_t *pass_lds() {
+    extern __shared__ __align__(16) uint8_t s_lds[];
+    return s_lds;
+}
+
+// LH: the histogram is in LDS (else straight to global memory)
+template <bool LH>
+__device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, double z,
                                             double (&v)[4]) {
     v[0] += z;
     v[1] += z * z;
     const int b = hist_bin(z, c.e0, c.e1, c.inv_step, c.nb);
     if (b >= 0) {
-        atomicAdd(&hist[b], 1u);
+        if (LH)
+            atomicAdd(reinterpret_cast<uint32_t *>(pass_lds()) + b, 1u);
+        else
+            __hip_atomic_fetch_add(ghist + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         v[3] += 1.0;
     }
 }
@@ -352,10 +384,9 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, uint32_t *hist, do
 // One tile of the fused pass.  FX = kStdFlags is the common configuration
 // with the flags folded at compile time; FX = 0 handles any other.  Both
 // evaluate the same expressions, so a bunch gets the same bits from either.
-// `hist` is the LDS or global histogram, sP the LDS window of P.
-template <uint32_t FX>
-__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32_t *hist,
-                                               const double *sP, double *s_red) {
+// LH: histogram in LDS at offset 0; the window of P follows at byte hb.
+template <uint32_t FX, bool LH>
+__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, int hb, double *s_red) {
     PassCtx c;
     load_ctx(B, c);
     const Flags<FX> fl{c.flags};
@@ -369,31 +400,34 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
     const uint32_t tabs = (uint32_t)(B.turn_offset + sc.turn);
     const bool inj = fl.has(F_INJ);
     const bool qe = fl.has(F_QE);
-    const double *nrm = inj ? B.normals + sc.turn * n : nullptr;
+    const g_f64 *nrm = inj ? (const g_f64 *)(B.normals + sc.turn * n) : nullptr;
     const uint64_t seed = B.seed;
     const bool kick2 = c.pending_kick && c.nb >= 2;
-    const int wlo = sc.win_lo;
+    const int wlo = window_lo(sc.centre_bin, B.win, c.nb);
     const unsigned wpairs = (unsigned)max(B.win - 1, 0);   // pair starts inside the window
-    const double2 *__restrict__ P2 = reinterpret_cast<const double2 *>(B.P2);
-    double *__restrict__ zp = B.z;
-    double *__restrict__ ep = B.dE;
+    const double *sP = reinterpret_cast<const double *>(pass_lds() + hb);
+    const g_f64x2 *P2 = (const g_f64x2 *)B.P2;
+    g_f64 *zp = (g_f64 *)B.z;
+    g_f64 *ep = (g_f64 *)B.dE;
+    g_u32 *ghist = (g_u32 *)hist_of(B, sc.turn);
     double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
 
     int64_t p = p0 + threadIdx.x;
     double2 zn = make_double2(0.0, 0.0), en = make_double2(0.0, 0.0);
     if (p < pf) {
-        zn = *reinterpret_cast<const double2 *>(zp + 2 * p);
-        en = *reinterpret_cast<const double2 *>(ep + 2 * p);
+        zn = ld2((const g_f64x2 *)(zp + 2 * p));
+        en = ld2((const g_f64x2 *)(ep + 2 * p));
     }
     for (; p < pf; p += blockDim.x) {
         const double2 zz = zn, ee = en;
         const int64_t pn = p + blockDim.x;
         if (pn < pf) {   // prefetch the next pair
-            zn = *reinterpret_cast<const double2 *>(zp + 2 * pn);
-            en = *reinterpret_cast<const double2 *>(ep + 2 * pn);
+            zn = ld2((const g_f64x2 *)(zp + 2 * pn));
+            en = ld2((const g_f64x2 *)(ep + 2 * pn));
         }
-        // the previous turn's potential at (P[k], P[k+1]): from the LDS
-        // window, or one 16-byte global gather outside it
+        // the previous turn's potential at (P[k], P[k+1]): read from the LDS
+        // window (clamped index, always valid), then replaced by one 16-byte
+        // global gather for the few particles outside it
         int k0 = 0, k1 = 0;
         double w0 = 0.0, w1 = 0.0;
         double2 q0 = make_double2(0.0, 0.0), q1 = make_double2(0.0, 0.0);
@@ -401,10 +435,13 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
             interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
             interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
             const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
-            if (j0 < wpairs) q0 = make_double2(sP[j0], sP[j0 + 1]);
-            else q0 = P2[k0];
-            if (j1 < wpairs) q1 = make_double2(sP[j1], sP[j1 + 1]);
-            else q1 = P2[k1];
+            if (wpairs) {
+                const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
+                q0 = make_double2(sP[i0], sP[i0 + 1]);
+                q1 = make_double2(sP[i1], sP[i1 + 1]);
+            }
+            if (j0 >= wpairs) q0 = ld2(P2 + k0);
+            if (j1 >= wpairs) q1 = ld2(P2 + k1);
         }
         double r0 = 0.0, r1 = 0.0;
         if (qe) {
@@ -433,13 +470,13 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
         advance(c, fl, z0, e0, r0, d0);
         advance(c, fl, z1, e1, r1, d1);
         if (wake) {
-            bin_and_sum(c, hist, z0, v);
-            bin_and_sum(c, hist, z1, v);
+            bin_and_sum<LH>(c, ghist, z0, v);
+            bin_and_sum<LH>(c, ghist, z1, v);
         }
         v[2] += d0;
         v[2] += d1;
-        *reinterpret_cast<double2 *>(zp + 2 * p) = make_double2(z0, z1);
-        *reinterpret_cast<double2 *>(ep + 2 * p) = make_double2(e0, e1);
+        st2((g_f64x2 *)(zp + 2 * p), z0, z1);
+        st2((g_f64x2 *)(ep + 2 * p), e0, e1);
     }
     // the single particle of an odd bunch (last pair)
     if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1) {
@@ -452,7 +489,7 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
         e0 = apply_pending(c, z0, e0);
         double d0;
         advance(c, fl, z0, e0, r0, d0);
-        if (wake) bin_and_sum(c, hist, z0, v);
+        if (wake) bin_and_sum<LH>(c, ghist, z0, v);
         v[2] += d0;
         zp[i] = z0;
         ep[i] = e0;
@@ -467,26 +504,27 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, uint32
 }
 
 // Adds the LDS histogram of bunch B into this turn's global counts.
-__device__ inline void flush_hist(const BunchDev &B, const uint32_t *s_hist) {
+__device__ inline void flush_hist(const BunchDev &B) {
     if (!pass_hist_bytes(B)) return;
     __syncthreads();
-    uint32_t *g = hist_of(B, B.sc->turn);
+    const uint32_t *s_hist = reinterpret_cast<const uint32_t *>(pass_lds());
+    g_u32 *g = (g_u32 *)hist_of(B, B.sc->turn);
     for (int b = threadIdx.x; b < B.nbins; b += blockDim.x) {
         const uint32_t h = s_hist[b];
-        if (h) atomicAdd(&g[b], h);
+        if (h) __hip_atomic_fetch_add(g + b, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 // Clears the LDS histogram of bunch B and loads its potential window.
-__device__ inline void stage_bunch(const BunchDev &B, uint8_t *lds) {
+__device__ inline void stage_bunch(const BunchDev &B) {
     __syncthreads();   // the previous bunch's LDS is no longer read
     const int hb = pass_hist_bytes(B);
-    uint32_t *h = reinterpret_cast<uint32_t *>(lds);
+    uint32_t *h = reinterpret_cast<uint32_t *>(pass_lds());
     for (int b = threadIdx.x; b < hb / 4; b += blockDim.x) h[b] = 0u;
     const TurnScalars &sc = *B.sc;
     if (sc.pending_kick && B.win > 0) {
-        double *sP = reinterpret_cast<double *>(lds + hb);
-        const double *P = B.P + sc.win_lo;
+        double *sP = reinterpret_cast<double *>(pass_lds() + hb);
+        const g_f64 *P = (const g_f64 *)(B.P + window_lo(sc.centre_bin, B.win, B.nbins));
         for (int k = threadIdx.x; k < B.win; k += blockDim.x) sP[k] = P[k];
     }
     __syncthreads();
@@ -500,27 +538,28 @@ template <uint32_t FX>
 __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bds,
                                                  const int32_t *__restrict__ tile_bunch,
                                                  int ntiles) {
-    extern __shared__ __align__(16) uint8_t s_lds[];
     __shared__ double s_red[4 * 16];
+    // kStdFlags has the wake on and F_GHIST off: its histogram is in LDS
+    constexpr bool kLdsOnly = (FX & F_WAKE) && !(FX & F_GHIST);
     const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per;
     const int t1 = min(ntiles, t0 + per);
-    uint32_t *s_hist = reinterpret_cast<uint32_t *>(s_lds);
     int staged = -1;
     for (int tile = t0; tile < t1; ++tile) {
         const int bi = tile_bunch[tile];
         const BunchDev &B = bds[bi];
         if (bi != staged) {
-            if (staged >= 0) flush_hist(bds[staged], s_hist);
-            stage_bunch(B, s_lds);
+            if (staged >= 0) flush_hist(bds[staged]);
+            stage_bunch(B);
             staged = bi;
         }
         const int hb = pass_hist_bytes(B);
-        uint32_t *hist = hb ? s_hist : hist_of(B, B.sc->turn);
-        turn_pass_tile<FX>(B, tile - B.tile_begin, hist,
-                           reinterpret_cast<const double *>(s_lds + hb), s_red);
+        if (kLdsOnly || hb)
+            turn_pass_tile<FX, true>(B, tile - B.tile_begin, hb, s_red);
+        else
+            turn_pass_tile<FX, false>(B, tile - B.tile_begin, hb, s_red);
     }
-    if (staged >= 0) flush_hist(bds[staged], s_hist);
+    if (staged >= 0) flush_hist(bds[staged]);
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__restrict__ bds,
@@ -548,10 +587,12 @@ __global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
 
 struct BinScalars {
     double sz, cur, nh;
+    int centre;   // interpolation cell of the centroid
 };
 
-// sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386) and the
-// binned count from the tile sums; valid in every thread
+// sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386), the
+// binned count and the centroid's cell from the tile sums; valid in every
+// thread.  s_bc holds 4 doubles.
 __device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, double *s_bc) {
     const TurnScalars &sc = *B.sc;
     double v[3] = {0.0, 0.0, 0.0};
@@ -569,9 +610,12 @@ __device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, doubl
                       : (((B.pf / sc.E0) / (kTwoPi * B.radius)) * sz) /
                             (sc.eta0 * (B.sigma_E0 * B.sigma_E0));
         s_bc[2] = v[2];
+        const double kc = floor((v[0] / (double)B.n - B.c_first) * B.inv_s);
+        const double top = (double)(B.nbins - 1);
+        s_bc[3] = (kc == kc) ? (kc < 0.0 ? 0.0 : (kc > top ? top : kc)) : 0.0;
     }
     __syncthreads();
-    return BinScalars{s_bc[0], s_bc[1], s_bc[2]};
+    return BinScalars{s_bc[0], s_bc[1], s_bc[2], (int)s_bc[3]};
 }
 
 __device__ __forceinline__ double line_density(const BunchDev &B, const uint32_t *h, int i,
@@ -637,7 +681,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fused(const BunchDev *__r
                                                              const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
     __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[3];
+    __shared__ double s_bc[4];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int o = items[2 * blockIdx.x + 1];
     const BinScalars bs = bin_scalars(B, s_red, s_bc);
@@ -645,6 +689,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fused(const BunchDev *__r
         B.sc->sigma_z = bs.sz;
         B.sc->current = bs.cur;
         B.sc->n_h = bs.nh;
+        B.sc->centre_bin = bs.centre;
     }
     clear_next_hist(B, o);
     if (bs.nh == 0.0 || B.nJ == 0) {   // wakefield.jl:305-311
@@ -665,7 +710,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
                                                            const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
     __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[3];
+    __shared__ double s_bc[4];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int a = items[2 * blockIdx.x + 1];
     const BinScalars bs = bin_scalars(B, s_red, s_bc);
@@ -673,6 +718,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
         B.sc->sigma_z = bs.sz;
         B.sc->current = bs.cur;
         B.sc->n_h = bs.nh;
+        B.sc->centre_bin = bs.centre;
     }
     clear_next_hist(B, a);
     if (bs.nh == 0.0) return;
@@ -688,7 +734,7 @@ __global__ __launch_bounds__(kConvInvThreads) void k_conv_inv(const BunchDev *__
                                                               const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
     __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[3];
+    __shared__ double s_bc[4];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int o = items[2 * blockIdx.x + 1];
     const BinScalars bs = bin_scalars(B, s_red, s_bc);
@@ -746,67 +792,33 @@ __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__res
 // E0 update support, moments, init, finalize
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kPassThreads) void k_interp_sum(const BunchDev *__restrict__ bds,
-                                                             const int32_t *__restrict__ tile_bunch) {
-    __shared__ double s_red[16];
-    const int tile = blockIdx.x;
-    const BunchDev &B = bds[tile_bunch[tile]];
-    if (!((B.flags & F_WAKE) && (B.flags & F_E0)) || (B.flags & F_PLDS)) return;
-    const int lt = tile - B.tile_begin;
-    const int64_t n = B.n;
-    const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
-    const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
-    const int64_t full = n >> 1;
-    const int64_t pf = min(p1, full);
-    const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
-    const int nb = B.nbins;
-    const double *__restrict__ P = B.P;
-    const double *__restrict__ zp = B.z;
-    double v[1] = {0.0};
-    if (nb >= 2) {
-        int64_t p = p0 + threadIdx.x;
-        double2 zn = make_double2(0.0, 0.0);
-        if (p < pf) zn = *reinterpret_cast<const double2 *>(zp + 2 * p);
-        for (; p < pf; p += blockDim.x) {
-            const double2 zz = zn;
-            if (p + blockDim.x < pf) zn = *reinterpret_cast<const double2 *>(zp + 2 * (p + blockDim.x));
-            int k0, k1;
-            double w0, w1;
-            interp_index(zz.x, c0, c1, inv, nb, k0, w0);
-            interp_index(zz.y, c0, c1, inv, nb, k1, w1);
-            v[0] += P[k0] * (1.0 - w0) + P[k0 + 1] * w0;
-            v[0] += P[k1] * (1.0 - w1) + P[k1 + 1] * w1;
-        }
-    } else {
-        int64_t p = p0 + threadIdx.x;
-        for (; p < pf; p += blockDim.x) v[0] += P[0] + P[0];
-    }
-    if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1)
-        v[0] += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
-    block_sum<1>(v, s_red);
-    if (threadIdx.x == 0) B.ipart[lt] = v[0];
-}
-
-// The same sums for bunches whose potential fits in LDS (F_PLDS): each
-// workgroup walks a contiguous range of tiles, loads the table once per bunch
-// it meets, and the per-particle gathers become LDS reads.  The per-tile sums
-// depend on the tile and this kernel's block size only.
-__global__ __launch_bounds__(kInterpThreads) void k_interp_sum_lds(
+// Sum over a tile of the interpolated potential at the post-drift z (the
+// E0 update of a wake bunch).  Each workgroup walks a contiguous range of
+// tiles and stages, once per bunch it meets, a window of kInterpWindow
+// entries of P around the centroid in LDS; the few particles outside it read
+// P from global memory.  Each thread adds its pairs in ascending order, so
+// the per-tile sums depend on the tile and the block size only.
+__global__ __launch_bounds__(kInterpThreads) void k_interp_sum(
     const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch, int ntiles) {
-    extern __shared__ double sP[];
+    __shared__ double sP[kInterpWindow];
     __shared__ double s_red[16];
     const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per;
     const int t1 = min(ntiles, t0 + per);
-    int staged = -1;
+    int staged = -1, wlo = 0;
+    unsigned wpairs = 0;
     for (int tile = t0; tile < t1; ++tile) {
         const int bi = tile_bunch[tile];
         const BunchDev &B = bds[bi];
-        if (!((B.flags & F_WAKE) && (B.flags & F_E0) && (B.flags & F_PLDS))) continue;
+        if (!((B.flags & F_WAKE) && (B.flags & F_E0))) continue;
         const int nb = B.nbins;
+        const double *__restrict__ P = B.P;
         if (bi != staged) {
+            const int w = min(nb, kInterpWindow);
+            wlo = window_lo(B.sc->centre_bin, w, nb);
+            wpairs = (unsigned)(w - 1);
             __syncthreads();
-            for (int k = threadIdx.x; k < nb; k += blockDim.x) sP[k] = B.P[k];
+            for (int k = threadIdx.x; k < w; k += blockDim.x) sP[k] = P[wlo + k];
             __syncthreads();
             staged = bi;
         }
@@ -817,19 +829,20 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_sum_lds(
         const int64_t full = n >> 1;
         const int64_t pf = min(p1, full);
         const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
-        const double *__restrict__ zp = B.z;
+        const g_f64 *zp = (const g_f64 *)B.z;
+        const g_f64 *gP = (const g_f64 *)P;
         double v[1] = {0.0};
         if (nb >= 2) {
-            // four pairs per thread in flight; each thread still adds its
-            // pairs in ascending order
+            // four pairs per thread in flight; P from the LDS window (clamped
+            // index), replaced by global reads outside it
             const int64_t bs = blockDim.x;
             for (int64_t p = p0 + threadIdx.x; p < pf; p += 4 * bs) {
                 double2 zz[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    zz[u] = (p + u * bs < pf)
-                                ? *reinterpret_cast<const double2 *>(zp + 2 * (p + u * bs))
-                                : make_double2(0.0, 0.0);
+                for (int u = 0; u < 4; ++u) {
+                    zz[u] = make_double2(0.0, 0.0);
+                    if (p + u * bs < pf) zz[u] = ld2((const g_f64x2 *)(zp + 2 * (p + u * bs)));
+                }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     if (p + u * bs >= pf) break;
@@ -837,15 +850,20 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_sum_lds(
                     double w0, w1;
                     interp_index(zz[u].x, c0, c1, inv, nb, k0, w0);
                     interp_index(zz[u].y, c0, c1, inv, nb, k1, w1);
-                    v[0] += sP[k0] * (1.0 - w0) + sP[k0 + 1] * w0;
-                    v[0] += sP[k1] * (1.0 - w1) + sP[k1 + 1] * w1;
+                    const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
+                    const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
+                    double a0 = sP[i0], b0 = sP[i0 + 1], a1 = sP[i1], b1 = sP[i1 + 1];
+                    if (j0 >= wpairs) { a0 = gP[k0]; b0 = gP[k0 + 1]; }
+                    if (j1 >= wpairs) { a1 = gP[k1]; b1 = gP[k1 + 1]; }
+                    v[0] += a0 * (1.0 - w0) + b0 * w0;
+                    v[0] += a1 * (1.0 - w1) + b1 * w1;
                 }
             }
         } else {
-            for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) v[0] += sP[0] + sP[0];
+            for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) v[0] += P[0] + P[0];
         }
         if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1)
-            v[0] += interp_pot(sP, zp[n - 1], c0, c1, inv, nb);
+            v[0] += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
         block_sum<1>(v, s_red);
         if (threadIdx.x == 0) B.ipart[lt] = v[0];
         __syncthreads();   // s_red is reused by the next tile
@@ -929,37 +947,27 @@ __global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restri
         s.pending_m = 0;
         s.drift_skipped_count = 0;
         s.turn = 0;
-        s.win_lo = 0;
+        s.centre_bin = 0;
         *B.sc = s;
     }
 }
 
 __global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__restrict__ bds,
                                                             const int32_t *__restrict__ list) {
-    __shared__ double s_red[3 * 16];
+    __shared__ double s_red[2 * 16];
     const BunchDev &B = bds[list[blockIdx.x]];
     const uint32_t fl = B.flags;
-    double v[3] = {0.0, 0.0, 0.0};   // sum (dE'-dE_init), sum interpolated P, sum z
-    for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
-        if (fl & F_E0) {
+    double v[2] = {0.0, 0.0};   // sum (dE'-dE_init), sum interpolated P
+    if (fl & F_E0) {
+        for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
             v[0] += B.part[4 * t + 2];
             if (fl & F_WAKE) v[1] += B.ipart[t];
         }
-        v[2] += B.part[4 * t];
     }
-    block_sum<3>(v, s_red);
+    block_sum<2>(v, s_red);
     if (threadIdx.x == 0) {
         TurnScalars s = *B.sc;
         double E0 = s.E0;
-        // the next pass's LDS window of P, centred on this turn's centroid
-        // (where the window sits changes no result, only where P is read)
-        s.win_lo = 0;
-        if ((fl & F_WAKE) && B.win > 0 && B.win < B.nbins && B.n > 0) {
-            const double kc = (v[2] / (double)B.n - B.c_first) * B.inv_s;
-            const double lo = floor(kc) - (double)(B.win / 2);
-            const double hi = (double)(B.nbins - B.win);
-            s.win_lo = (lo == lo) ? (int32_t)(lo < 0.0 ? 0.0 : (lo > hi ? hi : lo)) : 0;
-        }
         if (s.drift_skip) s.drift_skipped_count += 1;
         if (fl & F_E0) {                                  // evolution.jl:419-429
             const double mean = (v[0] - v[1]) / (double)B.n;
@@ -1081,6 +1089,137 @@ __global__ void k_generate(uint64_t seed, double mu_z, double sz, double mu_E, d
     dE[i] = mu_E + sE * b;
 }
 
+// generate_particles (evolution.jl:42-133) for n >= 2, step 1: one workgroup
+// forms the sample covariance of m = min(n, 10000) independent samples
+// (stream 2, regenerated from their counters for the second pass): mean-
+// centred sums with m-1 normalisation (:75-82), variances floored at 1e-18,
+// the diagonal lifted when the smallest eigenvalue is below 1e-12 (:89-94),
+// then the lower Cholesky factor of the MvNormal (:97-101) -> L[0..2].
+__device__ __forceinline__ void gen_sample(uint64_t seed, int64_t j, uint32_t stream, double &a,
+                                           double &b) {
+    uint32_t x[4];
+    philox4x32_10((uint32_t)j, (uint32_t)((uint64_t)j >> 32), 0u, stream, (uint32_t)seed,
+                  (uint32_t)(seed >> 32), x);
+    box_muller(x, a, b);
+}
+
+__global__ __launch_bounds__(1024) void k_gen_cov(uint64_t seed, double mu_z, double mu_E,
+                                                  double sz, double sE, int64_t m,
+                                                  double *__restrict__ L) {
+    __shared__ double s_red[3 * 16];
+    __shared__ double s_mean[2];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int64_t j = threadIdx.x; j < m; j += blockDim.x) {
+        double a, b;
+        gen_sample(seed, j, 2u, a, b);
+        v[0] += mu_z + sz * a;
+        v[1] += mu_E + sE * b;
+    }
+    double w[2] = {v[0], v[1]};
+    block_sum<2>(w, s_red);
+    if (threadIdx.x == 0) {
+        s_mean[0] = w[0] / (double)m;
+        s_mean[1] = w[1] / (double)m;
+    }
+    __syncthreads();
+    const double mz = s_mean[0], me = s_mean[1];
+    v[0] = v[1] = v[2] = 0.0;
+    for (int64_t j = threadIdx.x; j < m; j += blockDim.x) {
+        double a, b;
+        gen_sample(seed, j, 2u, a, b);
+        const double dz = (mu_z + sz * a) - mz, de = (mu_E + sE * b) - me;
+        v[0] += dz * dz;
+        v[1] += de * de;
+        v[2] += dz * de;
+    }
+    block_sum<3>(v, s_red);
+    if (threadIdx.x == 0) {
+        const double md = (double)(m - 1);
+        double a = v[0] / md, c = v[1] / md;
+        const double b = v[2] / md;
+        a = a < 1e-18 ? 1e-18 : a;
+        c = c < 1e-18 ? 1e-18 : c;
+        // smallest eigenvalue as det / largest: no cancellation when a << c
+        const double h = (a - c) * 0.5;
+        const double lmax = (a + c) * 0.5 + sqrt(h * h + b * b);
+        const double lmin = (a * c - b * b) / lmax;
+        if (lmin < 1e-12) {
+            const double lift = 1e-12 - lmin;
+            a += lift;
+            c += lift;
+        }
+        const double l11 = sqrt(a);
+        const double l21 = b / l11;
+        L[0] = l11;
+        L[1] = l21;
+        L[2] = sqrt(c - l21 * l21);
+    }
+}
+
+// step 2: particle i = mu + L xi, xi the stream-1 normals of particle i
+__global__ void k_gen_mvn(uint64_t seed, double mu_z, double mu_E, const double *__restrict__ L,
+                          double *__restrict__ z, double *__restrict__ dE, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double l11 = L[0], l21 = L[1], l22 = L[2];
+    double x1, x2;
+    gen_sample(seed, i, 1u, x1, x2);
+    z[i] = mu_z + l11 * x1;
+    dE[i] = mu_E + (l21 * x1 + l22 * x2);
+}
+
+// step 3: compute_std (utils.jl:531-544) of z and dE in two passes.  Block b
+// sums [b*chunk, (b+1)*chunk); mode 0 sums the values, mode 1 the squared
+// deviations from stats[0..1].
+__global__ __launch_bounds__(1024) void k_gen_sums(int mode, const double *__restrict__ z,
+                                                   const double *__restrict__ dE, int64_t n,
+                                                   int64_t chunk, const double *__restrict__ stats,
+                                                   double *__restrict__ part) {
+    __shared__ double s_red[2 * 16];
+    const int64_t i0 = (int64_t)blockIdx.x * chunk;
+    const int64_t i1 = min(n, i0 + chunk);
+    double v[2] = {0.0, 0.0};
+    const double cz = mode ? stats[0] : 0.0, ce = mode ? stats[1] : 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        if (mode) {
+            const double a = z[i] - cz, b = dE[i] - ce;
+            v[0] += a * a;
+            v[1] += b * b;
+        } else {
+            v[0] += z[i];
+            v[1] += dE[i];
+        }
+    }
+    block_sum<2>(v, s_red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// mode 0: stats[0..1] = means; mode 1: stats[2..3] = sample standard deviations
+__global__ __launch_bounds__(256) void k_gen_sums_final(int mode, const double *__restrict__ part,
+                                                        int nblocks, int64_t n,
+                                                        double *__restrict__ stats) {
+    __shared__ double s_red[2 * 16];
+    double v[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        v[0] += part[2 * b];
+        v[1] += part[2 * b + 1];
+    }
+    block_sum<2>(v, s_red);
+    if (threadIdx.x == 0) {
+        if (mode == 0) {
+            stats[0] = v[0] / (double)n;
+            stats[1] = v[1] / (double)n;
+        } else {
+            const double vz = v[0] / (double)(n - 1), ve = v[1] / (double)(n - 1);
+            stats[2] = sqrt(vz > 0.0 ? vz : 0.0);
+            stats[3] = sqrt(ve > 0.0 ? ve : 0.0);
+        }
+    }
+}
+
 // Plain device copy (16 B per lane per step): the HBM ceiling of a kernel
 // that reads and writes each byte once, and the calibration of the HBM
 // byte counters.
@@ -1101,8 +1240,7 @@ hipError_t configure_kernels(int lds) {
     const void *fns[] = {(const void *)k_turn_pass, (const void *)k_turn_pass_std,
                          (const void *)k_conv_fused,
                          (const void *)k_conv_fwd, (const void *)k_conv_inv,
-                         (const void *)k_fft_wake, (const void *)k_stage_hist,
-                         (const void *)k_interp_sum_lds};
+                         (const void *)k_fft_wake, (const void *)k_stage_hist};
     hipError_t last = hipSuccess;
     for (const void *f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -1163,16 +1301,10 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
     return hipGetLastError();
 }
 
-hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, int ntiles, hipStream_t s) {
-    if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_interp_sum, dim3(ntiles), dim3(kPassThreads), 0, s, bds, tb);
-    return hipGetLastError();
-}
-
-hipError_t launch_interp_sum_lds(const BunchDev *bds, const int32_t *tb, int ntiles, int grid,
-                                 size_t lds, hipStream_t s) {
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, int ntiles, int grid,
+                             hipStream_t s) {
     if (ntiles <= 0 || grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_interp_sum_lds, dim3((unsigned)grid), dim3(kInterpThreads), lds, s, bds, tb,
+    hipLaunchKernelGGL(k_interp_sum, dim3((unsigned)grid), dim3(kInterpThreads), 0, s, bds, tb,
                        ntiles);
     return hipGetLastError();
 }
@@ -1248,6 +1380,27 @@ hipError_t launch_generate(uint64_t seed, double mu_z, double sz, double mu_E, d
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_generate, dim3(blocks_for(n, 256)), dim3(256), 0, s, seed, mu_z, sz, mu_E,
                        sE, z, dE, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate_particles(uint64_t seed, double mu_z, double mu_E, double sz, double sE,
+                                     double *z, double *dE, int64_t n, double *work,
+                                     hipStream_t s) {
+    if (n < 2) return hipSuccess;
+    double *L = work, *stats = work + 4, *part = work + 8;
+    hipLaunchKernelGGL(k_gen_cov, dim3(1), dim3(1024), 0, s, seed, mu_z, mu_E, sz, sE,
+                       n < 10000 ? n : (int64_t)10000, L);
+    hipLaunchKernelGGL(k_gen_mvn, dim3(blocks_for(n, 256)), dim3(256), 0, s, seed, mu_z, mu_E,
+                       (const double *)L, z, dE, n);
+    const int64_t nbl = (n + 1023) / 1024;
+    const int nb = (int)(nbl < kGenSumBlocks ? nbl : kGenSumBlocks);
+    const int64_t chunk = (n + nb - 1) / nb;
+    for (int mode = 0; mode < 2; ++mode) {
+        hipLaunchKernelGGL(k_gen_sums, dim3(nb), dim3(1024), 0, s, mode, (const double *)z,
+                           (const double *)dE, n, chunk, (const double *)stats, part);
+        hipLaunchKernelGGL(k_gen_sums_final, dim3(1), dim3(256), 0, s, mode, (const double *)part,
+                           nb, n, stats);
+    }
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@ constexpr int kConvThreads = 256;      // convolution block
 constexpr int kConvInvThreads = 1024;  // phase-B convolution block (spectral sums in flight)
 constexpr int kFinalThreads = 256;     // per-bunch finalize block
 constexpr int kInterpThreads = 1024;   // LDS-staged interpolation sums
+constexpr int kInterpWindow = 8192;    // entries of P they stage (64 KiB: two workgroups per CU)
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (128 KiB of complex FP64)
 constexpr int kAutoFFTBlock = 1024;    // default convolution block (F = 2048)
 
@@ -38,12 +39,10 @@ hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, int 
 // items are (bunch, block) pairs; Fmax is the largest FFT length among them
 hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
                        hipStream_t s);
-hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
+// interpolation sums of the wake + update_E0 bunches: `grid` workgroups over
+// the tile table
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int grid,
                              hipStream_t s);
-// the F_PLDS bunches' interpolation sums: `grid` workgroups over the tile
-// table, lds_bytes = 8 x the largest bin count among them
-hipError_t launch_interp_sum_lds(const BunchDev *bds, const int32_t *tile_bunch, int ntiles,
-                                 int grid, size_t lds_bytes, hipStream_t s);
 hipError_t launch_finalize(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
 // FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
 hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,
@@ -67,6 +66,14 @@ hipError_t launch_generate(uint64_t seed, double mu_z, double sz, double mu_E, d
                            double *z, double *dE, int64_t n, hipStream_t s);
 hipError_t launch_turn_scalars(const BunchDev *bd_dev, double E0, TurnScalars *out,
                                hipStream_t s);
+// generate_particles for n >= 2 (sample-covariance MvNormal): writes z, dE and
+// work[0..2] = Cholesky factor (l11, l21, l22), work[4..7] = (mean z, mean dE,
+// std z, std dE); work holds kGenWorkDoubles doubles
+constexpr int kGenSumBlocks = 1024;
+constexpr int kGenWorkDoubles = 8 + 2 * kGenSumBlocks;
+hipError_t launch_generate_particles(uint64_t seed, double mu_z, double mu_E, double sz, double sE,
+                                     double *z, double *dE, int64_t n, double *work,
+                                     hipStream_t s);
 // device-to-device copy of n doubles (n even) by a plain grid-stride kernel
 hipError_t launch_stream_copy(const double *src, double *dst, int64_t n, int blocks, hipStream_t s);
 

@@ -33,7 +33,6 @@ enum BunchFlags : uint32_t {
     F_INJ = 32u,       // injected normals instead of Philox
     F_CHARGE = 64u,    // fixed wake current (charge extension / stage entry)
     F_GHIST = 128u,    // histogram straight to global memory (too many bins for LDS)
-    F_PLDS = 256u,     // the potential fits in LDS: interpolation sums read it there
 };
 
 // the all-physics-on configuration the turn pass is specialised for, and the
@@ -59,7 +58,8 @@ struct TurnScalars {
     int32_t drift_skip;     // apply_phase_advance! skip condition of this turn
     int32_t drift_skipped_count;
     int64_t turn;           // turns completed in this call
-    int32_t win_lo;         // first bin of the pass's LDS window of P (around the centroid)
+    int32_t centre_bin;     // interpolation cell of this turn's centroid: the LDS windows
+                            // of P are placed around it (placement changes no result)
     int32_t pad_;
 };
 
@@ -97,9 +97,15 @@ struct BunchDev {
 
 // LDS of a turn-pass workgroup for one bunch: its histogram (nbins counts,
 // unless F_GHIST), then a window of `win` entries of the previous turn's
-// potential, P[win_lo .. win_lo+win), for the wake-kick gathers.
+// potential, P[lo .. lo+win), for the wake-kick gathers.
 __host__ __device__ inline int pass_hist_bytes(const BunchDev &B) {
     return ((B.flags & F_WAKE) && !(B.flags & F_GHIST)) ? ((B.nbins * 4 + 15) & ~15) : 0;
+}
+
+// first entry of a window of w entries of P centred on `centre`
+__host__ __device__ inline int window_lo(int centre, int w, int nbins) {
+    const int lo = centre - w / 2;
+    return lo < 0 ? 0 : (lo > nbins - w ? nbins - w : lo);
 }
 
 }  // namespace hm

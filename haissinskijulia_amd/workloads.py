@@ -94,3 +94,47 @@ WORKLOADS = {
     "ring3_1e8": dict(n=100_000_000, nbins=16384, half_width=8.0, charge=10e-9, turns=5000,
                       seed=11, resonators=[RESONATOR_BB, RESONATOR_LOWQ]),
 }
+
+
+def scan_bunches(count: int = 256, seed: int = 3) -> list[dict]:
+    """The bunches of a batched parameter scan on the 3 GeV ring: particle
+    counts log-uniform in [1e4, 4e6], V_rf uniform in [2.5, 4.0] MV, charge
+    uniform in [0.5, 10] nC, 4096 bins over +-8 sigma_z, the broadband
+    resonator; per-bunch seeds spawned from numpy.random.SeedSequence(seed)."""
+    import numpy as np
+    ss = np.random.SeedSequence(seed)
+    rng = np.random.default_rng(ss)
+    n = np.exp(rng.uniform(math.log(1e4), math.log(4e6), count)).astype(np.int64)
+    V = rng.uniform(2.5e6, 4.0e6, count)
+    q = rng.uniform(0.5e-9, 10e-9, count)
+    seeds = [int(c.generate_state(1, np.uint64)[0]) for c in ss.spawn(count)]
+    return [dict(n=int(n[i]), V=float(V[i]), charge=float(q[i]), seed=seeds[i], nbins=4096,
+                 half_width=8.0, resonators=[RESONATOR_BB]) for i in range(count)]
+
+
+def serve_jobs(count: int, rate: float, n_range=(1e4, 1e7), turn_range=(1e3, 2e4),
+               fd_fraction: float = 0.2, seed: int = 5) -> list[dict]:
+    """A stream of tracking jobs on the 3 GeV ring: Poisson arrivals at `rate`
+    jobs/s, particle counts and turn counts log-uniform in the given ranges,
+    V_rf = 3.2 MV.  About `fd_fraction` of the jobs come as finite-difference
+    pairs (V_rf x (1 -+ 0.5%)) with common random numbers: both members have
+    the same arrival, particle seed and QE seed.  1024 bins over +-8 sigma_z,
+    1 nC, the broadband resonator."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs: list[dict] = []
+    t = 0.0
+    while len(jobs) < count:
+        t += rng.exponential(1.0 / rate)
+        n = int(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1]))))
+        turns = int(math.exp(rng.uniform(math.log(turn_range[0]), math.log(turn_range[1]))))
+        s = int(rng.integers(1, 2 ** 62))
+        base = dict(arrival=t, n=n, turns=turns, seed=s, V=3.2e6, charge=1e-9, nbins=1024,
+                    half_width=8.0, resonators=[RESONATOR_BB], pair=-1)
+        if len(jobs) + 1 < count and rng.uniform() < fd_fraction / 2:
+            k = len(jobs)
+            jobs.append(dict(base, V=3.2e6 * (1 - 0.005), pair=k + 1))
+            jobs.append(dict(base, V=3.2e6 * (1 + 0.005), pair=k))
+        else:
+            jobs.append(base)
+    return jobs

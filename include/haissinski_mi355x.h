@@ -111,6 +111,16 @@ hm_status hm_bunch_download(hm_bunch *b, double *z, double *dE);
  * the Philox stream: z ~ N(mu_z, sigma_z), dE ~ N(mu_E, sigma_E). */
 hm_status hm_bunch_generate(hm_bunch *b, uint64_t seed, double mu_z, double sigma_z,
                             double mu_E, double sigma_E);
+/* generate_particles (src/evolution.jl:42-133) into the bunch: for n >= 2 the
+ * sample covariance of min(n, 10000) independent draws (variances floored at
+ * 1e-18, diagonal lifted to a smallest eigenvalue of 1e-12) and a MvNormal
+ * bunch mu + L xi from it; n == 1 draws independent normals; n == 0 draws
+ * nothing.  Returns the bunch's compute_std spreads (src/utils.jl:531-544), the
+ * input spreads for n == 1 and 0 for n == 0, as the reference does.  Draws
+ * come from the Philox stream of `seed`, not Julia's default RNG. */
+hm_status hm_bunch_generate_particles(hm_bunch *b, uint64_t seed, double mu_z, double mu_E,
+                                      double sigma_z, double sigma_E, double *sigma_E_out,
+                                      double *sigma_z_out);
 /* longitudinal_evolve! on a resident bunch (no host transfer of particles). */
 hm_status hm_bunch_evolve(hm_bunch *b, int64_t nbins, const hm_params *params,
                           const hm_options *opts, hm_result *result);

@@ -415,6 +415,109 @@ void or_generate_gaussian(uint64_t seed, double mu_z, double sigma_z, double mu_
     }
 }
 
+/* evolution.jl:42-133 (generate_particles) on the Philox streams.
+ *  n <= 0: nothing drawn; spreads 0 (:47-53).
+ *  n == 1: independent normals, the particle-1 block of stream 1; the
+ *          returned spreads are the inputs (:61-66).
+ *  n >= 2: m = min(n, 10000) independent samples (:68-72), sample j from the
+ *          block {j lo, j hi, 0, 2}; their sample covariance (Statistics.cov:
+ *          mean-centred, m-1 normalisation) with variances floored at 1e-18
+ *          (:75-82); the diagonal lifted by 1e-12 - lambda_min when the
+ *          smallest eigenvalue is below 1e-12 (:89-94); particle i is
+ *          mu + L xi with L the lower Cholesky factor of that matrix and xi
+ *          the stream-1 block of particle i (MvNormal, :97-113); the returned
+ *          spreads are compute_std of the bunch (utils.jl:531-544).
+ * L_out (3 doubles: l11, l21, l22) may be NULL. */
+static double seq_mean(const double *x, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += x[i];
+    return s / (double)n;
+}
+
+static double compute_std(const double *x, int64_t n) {   /* utils.jl:531-544 */
+    if (n <= 1) return 0.0;
+    const double mu = seq_mean(x, n);
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double d = x[i] - mu;
+        s += d * d;
+    }
+    const double var = s / (double)(n - 1);
+    return sqrt(var > 0.0 ? var : 0.0);
+}
+
+void or_generate_particles(uint64_t seed, double mu_z, double mu_E, double sigma_z, double sigma_E,
+                           int64_t n, double *z, double *dE, double *sigma_E_out,
+                           double *sigma_z_out, double *L_out) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    if (n <= 0) {
+        *sigma_E_out = 0.0;
+        *sigma_z_out = 0.0;
+        return;
+    }
+    if (n == 1) {
+        or_generate_gaussian(seed, mu_z, sigma_z, mu_E, sigma_E, 1, z, dE);
+        *sigma_E_out = sigma_E;
+        *sigma_z_out = sigma_z;
+        return;
+    }
+    const int64_t m = n < 10000 ? n : 10000;
+    double *zs = (double *)malloc(sizeof(double) * (size_t)m);
+    double *es = (double *)malloc(sizeof(double) * (size_t)m);
+    for (int64_t j = 0; j < m; ++j) {
+        uint32_t ctr[4] = {(uint32_t)j, (uint32_t)((uint64_t)j >> 32), 0u, 2u};
+        uint32_t x[4];
+        or_philox4x32_10(ctr, key, x);
+        double a, b;
+        box_muller(x, &a, &b);
+        zs[j] = mu_z + sigma_z * a;
+        es[j] = mu_E + sigma_E * b;
+    }
+    const double mz = seq_mean(zs, m), me = seq_mean(es, m);
+    double szz = 0.0, see = 0.0, sze = 0.0;
+    for (int64_t j = 0; j < m; ++j) {
+        const double dz = zs[j] - mz, de = es[j] - me;
+        szz += dz * dz;
+        see += de * de;
+        sze += dz * de;
+    }
+    free(zs);
+    free(es);
+    const double md = (double)(m - 1);
+    double a = szz / md, c = see / md;
+    const double b = sze / md;
+    if (a < 1e-18) a = 1e-18;
+    if (c < 1e-18) c = 1e-18;
+    /* smallest eigenvalue as det / largest: no cancellation when a << c */
+    const double h = (a - c) * 0.5;
+    const double lmax = (a + c) * 0.5 + sqrt(h * h + b * b);
+    const double lmin = (a * c - b * b) / lmax;
+    if (lmin < 1e-12) {
+        const double lift = 1e-12 - lmin;
+        a += lift;
+        c += lift;
+    }
+    const double l11 = sqrt(a);
+    const double l21 = b / l11;
+    const double l22 = sqrt(c - l21 * l21);
+    if (L_out) {
+        L_out[0] = l11;
+        L_out[1] = l21;
+        L_out[2] = l22;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), 0u, 1u};
+        uint32_t x[4];
+        or_philox4x32_10(ctr, key, x);
+        double x1, x2;
+        box_muller(x, &x1, &x2);
+        z[i] = mu_z + l11 * x1;
+        dE[i] = mu_E + (l21 * x1 + l22 * x2);
+    }
+    *sigma_z_out = compute_std(z, n);
+    *sigma_E_out = compute_std(dE, n);
+}
+
 /* ---------------------------------------------------------------------------
  * evolution.jl:206-471, MPI branch with R rank-equivalents.  Rank r owns the
  * contiguous chunk given by benchmark_total.jl:178-189 (base + remainder to

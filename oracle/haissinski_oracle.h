@@ -109,6 +109,10 @@ void or_normals(uint64_t seed, int64_t turn, int64_t i0, int64_t n, double *out)
 /* initial bunch from the Philox stream (independent Gaussians) */
 void or_generate_gaussian(uint64_t seed, double mu_z, double sigma_z, double mu_E,
                           double sigma_E, int64_t n, double *z, double *dE);
+/* evolution.jl:42-133: sample-covariance MvNormal bunch on the Philox streams */
+void or_generate_particles(uint64_t seed, double mu_z, double mu_E, double sigma_z, double sigma_E,
+                           int64_t n, double *z, double *dE, double *sigma_E_out,
+                           double *sigma_z_out, double *L_out);
 /* evolution.jl:206-471, MPI branch with nranks rank-equivalents */
 int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *p,
               const or_options *o, or_result *res, double *P_last);

@@ -98,6 +98,8 @@ def lib():
         L.or_philox4x32_10.argtypes = [_up, _up, _up]
         L.or_normals.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, _dp]
         L.or_generate_gaussian.argtypes = [C.c_uint64] + [C.c_double] * 4 + [C.c_int64, _dp, _dp]
+        L.or_generate_particles.argtypes = [C.c_uint64] + [C.c_double] * 4 + \
+            [C.c_int64, _dp, _dp, _dp, _dp, _dp]
         L.or_evolve.restype = C.c_int
         L.or_evolve.argtypes = [_dp, _dp, C.c_int64, C.c_int64, C.POINTER(OrParams),
                                 C.POINTER(OrOptions), C.POINTER(OrResult), _dp]
@@ -242,6 +244,18 @@ def generate_gaussian(seed, mu_z, sigma_z, mu_E, sigma_E, n):
     z = np.empty(n); dE = np.empty(n)
     lib().or_generate_gaussian(seed, mu_z, sigma_z, mu_E, sigma_E, n, _d(z), _d(dE))
     return z, dE
+
+
+def generate_particles(seed, mu_z, mu_E, sigma_z, sigma_E, n):
+    """generate_particles (evolution.jl:42-133) on the Philox streams:
+    returns (z, dE, sigma_E, sigma_z, L) with L = (l11, l21, l22)."""
+    m = max(n, 1)
+    z = np.empty(m); dE = np.empty(m)
+    sE = np.zeros(1); sz = np.zeros(1); L = np.zeros(3)
+    lib().or_generate_particles(seed, mu_z, mu_E, sigma_z, sigma_E, n, _d(z), _d(dE), _d(sE),
+                                _d(sz), _d(L))
+    k = max(n, 0)
+    return z[:k], dE[:k], float(sE[0]), float(sz[0]), L
 
 
 # ---------------------------------------------------------------------------

@@ -113,6 +113,27 @@ def test_runs_are_bitwise_reproducible(dev):
     assert outs[0][2] == outs[1][2]
 
 
+@pytest.mark.parametrize("turns", [15, 16, 50, 161])
+def test_graph_replays_equal_direct_launches(dev, turns):
+    """Without timing, blocks of 16 turns run as replays of one captured HIP
+    graph; with timing every turn is launched directly.  Same bits."""
+    p = H.SimulationParameters.from_dict(W.ring_3gev_lattice(n_turns=turns))
+    z0, e0 = O.generate_gaussian(21, 0.0, 4e-3, 0.0, 3e6, 30_001)
+    outs = []
+    for timing in (True, False, False):
+        b = H.DeviceBunch(z0.size, dev)
+        b.upload(z0, e0)
+        r = b.evolve(p, 1024, H.TrackingOptions(seed=8, resonators=[W.RESONATOR_BB], charge=2e-9,
+                                                timing=timing))
+        outs.append((*b.download(), r))
+        b.close()
+    for z, e, r in outs[1:]:
+        np.testing.assert_array_equal(z, outs[0][0])
+        np.testing.assert_array_equal(e, outs[0][1])
+        assert (r.sigma_E, r.sigma_z, r.E0, r.turns_done) == \
+               (outs[0][2].sigma_E, outs[0][2].sigma_z, outs[0][2].E0, turns)
+
+
 def test_zero_turns_returns_initial_spreads(dev):
     p = W.ring_3gev_lattice(n_turns=0)
     z0, e0 = O.generate_gaussian(2, 0.0, 4e-3, 0.0, 3e6, 1000)

@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import POTENTIAL_RTOL, W
+from helpers import POTENTIAL_RTOL, W, normwise
 from oracle import oracle as O
 
 import haissinskijulia_amd as H
@@ -155,12 +155,42 @@ def test_device_normals_match_oracle_stream(dev):
 
 
 def test_device_generation_matches_oracle(dev):
-    p, sE, sz, E0 = H.generate_particles(0.0, 0.0, 4e-3, 3e6, 50_001, 3e9, H.MASS_ELECTRON,
-                                         2.8, 4.7e8, seed=11, comm=dev)
+    b = H.DeviceBunch(50_001, dev)
+    b.generate(11, 4e-3, 3e6)
+    zg, eg = b.download()
+    b.close()
     z, dE = O.generate_gaussian(11, 0.0, 4e-3, 0.0, 3e6, 50_001)
-    np.testing.assert_allclose(p.coordinates.z, z, rtol=4e-15, atol=1e-18)
-    np.testing.assert_allclose(p.coordinates.ΔE, dE, rtol=4e-15, atol=1e-8)
-    assert sz == pytest.approx(4e-3, rel=0.02) and sE == pytest.approx(3e6, rel=0.02)
+    np.testing.assert_allclose(zg, z, rtol=4e-15, atol=1e-18)
+    np.testing.assert_allclose(eg, dE, rtol=4e-15, atol=1e-8)
+
+
+@pytest.mark.parametrize("n", [2, 3, 9_999, 10_000, 10_001, 250_001])
+def test_generate_particles_matches_oracle(dev, n):
+    """generate_particles (evolution.jl:42-133): sample-covariance MvNormal.
+    The covariance sums are reduced in a different order on the GPU, so the
+    Cholesky factor and the coordinates agree to rounding, not bitwise."""
+    p, sE, sz, E0 = H.generate_particles(1e-4, -2e3, 4e-3, 3e6, n, 3e9, H.MASS_ELECTRON,
+                                         2.8, 4.7e8, seed=77, comm=dev)
+    z, dE, sEr, szr, L = O.generate_particles(77, 1e-4, -2e3, 4e-3, 3e6, n)
+    assert E0 == 3e9
+    if n >= 100:   # well-conditioned sample covariance
+        assert normwise(p.coordinates.z - 1e-4, z - 1e-4) <= 1e-12
+        assert normwise(p.coordinates.ΔE + 2e3, dE + 2e3) <= 1e-12
+        assert sz == pytest.approx(szr, rel=1e-12) and sE == pytest.approx(sEr, rel=1e-12)
+    else:          # m <= 3 samples: the eigenvalue lift is at rounding level
+        assert np.all(np.isfinite(p.coordinates.z)) and np.all(np.isfinite(p.coordinates.ΔE))
+        assert sz == pytest.approx(np.std(p.coordinates.z, ddof=1), rel=1e-12)
+
+
+def test_generate_particles_one_and_zero(dev):
+    p, sE, sz, E0 = H.generate_particles(0.0, 0.0, 4e-3, 3e6, 1, 3e9, H.MASS_ELECTRON, 2.8, 4.7e8,
+                                         seed=5, comm=dev)
+    z, dE = O.generate_gaussian(5, 0.0, 4e-3, 0.0, 3e6, 1)
+    assert (sE, sz) == (3e6, 4e-3)
+    np.testing.assert_allclose(p.coordinates.z, z, rtol=4e-15)
+    p, sE, sz, E0 = H.generate_particles(0.0, 0.0, 4e-3, 3e6, 0, 3e9, H.MASS_ELECTRON, 2.8, 4.7e8,
+                                         seed=5, comm=dev)
+    assert len(p) == 0 and (sE, sz, E0) == (0.0, 0.0, 3e9)
 
 
 def test_global_std_matches_oracle(dev):

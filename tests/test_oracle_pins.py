@@ -65,6 +65,46 @@ def test_generate_gaussian_uses_stream_one():
         assert dE[i] == pytest.approx(3e6 * b, rel=1e-15)
 
 
+@pytest.mark.parametrize("n", [777, 10_000, 25_000])
+def test_generate_particles_is_sample_covariance_mvnormal(n):
+    """evolution.jl:68-128 recomputed with numpy: the stream-2 samples, their
+    covariance (numpy.cov, ddof=1), eigvalsh and the lower Cholesky factor;
+    the bunch is mu + L xi with xi the stream-1 normals; the spreads are
+    numpy.std(ddof=1)."""
+    seed, mz, mE, sz, sE = 77, 1e-4, -2e3, 4e-3, 3e6
+    z, dE, sEo, szo, L = O.generate_particles(seed, mz, mE, sz, sE, n)
+    m = min(n, 10_000)
+    a = np.empty(m)
+    b = np.empty(m)
+    for j in range(m):
+        a[j], b[j] = _bm_from_words(O.philox4x32_10([j, 0, 0, 2], [seed, 0]))
+    S = np.cov(np.vstack([mz + sz * a, mE + sE * b]))
+    S[0, 0] = max(S[0, 0], 1e-18)
+    S[1, 1] = max(S[1, 1], 1e-18)
+    lmin = np.linalg.eigvalsh(S)[0]
+    if lmin < 1e-12:
+        S += (1e-12 - lmin) * np.eye(2)
+    Lr = np.linalg.cholesky(S)
+    np.testing.assert_allclose(L, [Lr[0, 0], Lr[1, 0], Lr[1, 1]], rtol=1e-12)
+    x1, x2 = O.generate_gaussian(seed, 0.0, 1.0, 0.0, 1.0, n)
+    np.testing.assert_allclose(z, mz + Lr[0, 0] * x1, rtol=1e-12, atol=1e-12 * sz)
+    np.testing.assert_allclose(dE, mE + (Lr[1, 0] * x1 + Lr[1, 1] * x2), rtol=1e-12,
+                               atol=1e-12 * sE)
+    assert szo == pytest.approx(np.std(z, ddof=1), rel=1e-13)
+    assert sEo == pytest.approx(np.std(dE, ddof=1), rel=1e-13)
+
+
+def test_generate_particles_small_counts():
+    z, dE, sEo, szo, L = O.generate_particles(5, 0.0, 0.0, 4e-3, 3e6, 0)
+    assert z.size == 0 and (sEo, szo) == (0.0, 0.0)
+    z, dE, sEo, szo, L = O.generate_particles(5, 0.0, 0.0, 4e-3, 3e6, 1)
+    z1, e1 = O.generate_gaussian(5, 0.0, 4e-3, 0.0, 3e6, 1)
+    assert (z[0], dE[0], sEo, szo) == (z1[0], e1[0], 3e6, 4e-3)
+    # two samples: rank-1 covariance, lifted to a smallest eigenvalue of ~1e-12
+    z, dE, sEo, szo, L = O.generate_particles(5, 0.0, 0.0, 4e-3, 3e6, 2)
+    assert np.all(np.isfinite(L)) and L[0] > 0
+
+
 # --------------------------------------------------------------------------
 # Bin grid (evolution.jl:270-274) -- exact rational rounding
 # --------------------------------------------------------------------------
