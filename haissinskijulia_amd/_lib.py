@@ -58,7 +58,8 @@ class HmResult(C.Structure):
         ("turns_done", C.c_int64), ("drift_skipped", C.c_int32), ("pad_", C.c_int32),
         ("ms_total", C.c_double), ("ms_pass_avg", C.c_double), ("ms_conv_avg", C.c_double),
         ("pass_launches", C.c_int64), ("sigma_E0", C.c_double), ("sigma_z0", C.c_double),
-        ("bin_half_width", C.c_double),
+        ("bin_half_width", C.c_double), ("mean_z", C.c_double), ("mean_dE", C.c_double),
+        ("corr_z_dE", C.c_double),
     ]
 
 
@@ -129,6 +130,7 @@ def load() -> C.CDLL:
             "hm_wake_potential": (st, [vp, _ip, C.c_int64, _dp, _dp, C.c_double, C.c_int32, _dp]),
             "hm_interpolate_kick": (st, [vp, _dp, _dp, C.c_int64, _dp, C.c_int64, _dp]),
             "hm_global_std": (st, [vp, _dp, C.c_int64, _dp]),
+            "hm_pair_stats": (st, [vp, _dp, _dp, C.c_int64, _dp]),
             "hm_normals": (st, [vp, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, _dp]),
             "hm_bin_grid": (st, [C.c_double, C.c_int64, _dp, _dp, _dp]),
             "hm_stream_copy": (st, [vp, C.c_int64, C.c_int32, _dp, _dp]),

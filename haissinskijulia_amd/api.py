@@ -201,12 +201,19 @@ class EvolveResult:
     sigma_E0: float
     sigma_z0: float
     bin_half_width: float
+    mean_z: float = 0.0
+    mean_dE: float = 0.0
+    corr_z_dE: float = 0.0    # Statistics.cor of the final bunch (two-pass, on the GPU)
 
     @classmethod
     def _from(cls, r: HmResult) -> "EvolveResult":
         return cls(r.sigma_E, r.sigma_z, r.E0, r.turns_done, r.drift_skipped, r.ms_total,
                    r.ms_pass_avg, r.ms_conv_avg, r.pass_launches, r.sigma_E0, r.sigma_z0,
-                   r.bin_half_width)
+                   r.bin_half_width, r.mean_z, r.mean_dE, r.corr_z_dE)
+
+    def __iter__(self):
+        """Unpacks like the reference's return value: (σ_E, σ_z, E0)."""
+        return iter((self.sigma_E, self.sigma_z, self.E0))
 
 
 # ---------------------------------------------------------------------------
@@ -410,6 +417,17 @@ class Server:
         check(_lib.load().hm_server_submit(self._h, C.byref(j), C.byref(jid)))
         return jid.value
 
+    def submit_voltage_pair(self, job: Job, rel_step: float = 0.005) -> tuple[int, int]:
+        """An RF-voltage sensitivity request: two jobs at V (1 -+ rel_step) with
+        common random numbers (same particle and QE seeds).  Returns their ids;
+        ``fd_sensitivity`` turns the two results into dFoM/dV."""
+        import dataclasses
+        lo = dataclasses.replace(job, params=job.params.replace(
+            voltage=job.params.voltage * (1.0 - rel_step)))
+        hi = dataclasses.replace(job, params=job.params.replace(
+            voltage=job.params.voltage * (1.0 + rel_step)))
+        return self.submit(lo), self.submit(hi)
+
     def cancel(self, job_id: int) -> None:
         check(_lib.load().hm_server_cancel(self._h, int(job_id)))
 
@@ -597,6 +615,68 @@ def compute_global_std(x: np.ndarray, comm=None) -> float:
     out = C.c_double()
     check(_lib.load().hm_global_std(_dev(comm).ctx, dptr(x), x.size, C.byref(out)))
     return out.value
+
+
+def pair_stats(z: np.ndarray, dE: np.ndarray, comm=None) -> dict:
+    """Two-pass mean, compute_std (src/utils.jl:531-544) and Statistics.cor of
+    (z, ΔE), reduced on the GPU."""
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    dE = np.ascontiguousarray(dE, dtype=np.float64)
+    if z.size != dE.size:
+        raise ValueError("z and ΔE differ in length")
+    out = np.zeros(5)
+    check(_lib.load().hm_pair_stats(_dev(comm).ctx, dptr(z), dptr(dE), z.size, dptr(out)))
+    return {"mean_z": out[0], "mean_dE": out[1], "std_z": out[2], "std_dE": out[3],
+            "cor": out[4]}
+
+
+# ---------------------------------------------------------------------------
+# figures of merit (src/parameter_sensitivity/types.jl, figures_of_merit.jl)
+# ---------------------------------------------------------------------------
+
+class FigureOfMerit:
+    """Abstract figure of merit (parameter_sensitivity/types.jl)."""
+
+
+class EnergySpreadFoM(FigureOfMerit):
+    pass
+
+
+class BunchLengthFoM(FigureOfMerit):
+    pass
+
+
+class EmittanceFoM(FigureOfMerit):
+    pass
+
+
+def compute_fom(fom: FigureOfMerit, particles: Particles, results, comm=None) -> float:
+    """compute_fom (src/parameter_sensitivity/figures_of_merit.jl:16-48):
+    σ_E, σ_z, or the emittance σ_z σ_E sqrt(1 - cor(z, ΔE)^2), with `results`
+    = (σ_E, σ_z, E0) of longitudinal_evolve.  The correlation is reduced on
+    the GPU; an EvolveResult of a resident bunch already carries it."""
+    σ_E, σ_z, _ = results
+    if isinstance(fom, EnergySpreadFoM):
+        return σ_E
+    if isinstance(fom, BunchLengthFoM):
+        return σ_z
+    if isinstance(fom, EmittanceFoM):
+        if isinstance(results, EvolveResult) and particles is None:
+            cor = results.corr_z_dE
+        else:
+            cor = pair_stats(particles.coordinates.z, particles.coordinates.ΔE, comm)["cor"]
+        return σ_z * σ_E * math.sqrt(1 - cor ** 2)
+    raise TypeError(f"unknown figure of merit {type(fom).__name__}")
+
+
+def fd_sensitivity(fom: FigureOfMerit, r_minus: EvolveResult, r_plus: EvolveResult,
+                   value: float, rel_step: float) -> float:
+    """Central finite difference d(FoM)/d(parameter) from a pair of runs at
+    value * (1 -+ rel_step) with common random numbers (same particle and QE
+    seeds), as the serving loop's sensitivity pairs produce."""
+    f_minus = compute_fom(fom, None, r_minus)
+    f_plus = compute_fom(fom, None, r_plus)
+    return (f_plus - f_minus) / (2.0 * value * rel_step)
 
 
 def stream_copy(nbytes: int, iters: int = 20, comm=None) -> tuple[float, float]:

@@ -326,9 +326,16 @@ hm_status Batch::run_subset_moments(const std::vector<int32_t> &which, int mode)
 hm_status Batch::epilogue(const std::vector<int32_t> &which) {
     if (which.empty()) return HM_OK;
     HM_TRY(run_subset_moments(which, 1));
+    hipStream_t s = ctx_->stream;
     for (int32_t i : which) {
-        rb_[(size_t)i].active = false;
-        rb_[(size_t)i].finished = true;
+        RunBunch &q = rb_[(size_t)i];
+        // two-pass means, spreads and cor(z, dE) of the final bunch -> out[9..13]
+        // (the emittance figure of merit, figures_of_merit.jl:36-47)
+        HM_TRY(q.b->w.stats.ensure((size_t)3 * kStatBlocks));
+        HIP_TRY(hipMemsetAsync(q.d.out + 9, 0, 5 * sizeof(double), s));
+        HIP_TRY(launch_pair_stats(q.b->z, q.b->dE, q.b->n, q.d.out + 9, q.b->w.stats.p, s));
+        q.active = false;
+        q.finished = true;
     }
     dirty_ = true;
     return HM_OK;
@@ -675,6 +682,9 @@ hm_status Batch::result(int i, hm_result *r, double *cz, double *cE) {
     r->ms_conv_avg = ms_conv();
     r->pass_launches = pass_count_;
     r->bin_half_width = (q.d.flags & F_WAKE) ? q.d.e_last : 0.0;
+    r->mean_z = o[9];
+    r->mean_dE = o[10];
+    r->corr_z_dE = o[13];
     if (cz) *cz = o[7];
     if (cE) *cE = o[8];
     return HM_OK;

@@ -1168,46 +1168,51 @@ __global__ void k_gen_mvn(uint64_t seed, double mu_z, double mu_E, const double 
     dE[i] = mu_E + (l21 * x1 + l22 * x2);
 }
 
-// step 3: compute_std (utils.jl:531-544) of z and dE in two passes.  Block b
-// sums [b*chunk, (b+1)*chunk); mode 0 sums the values, mode 1 the squared
-// deviations from stats[0..1].
-__global__ __launch_bounds__(1024) void k_gen_sums(int mode, const double *__restrict__ z,
-                                                   const double *__restrict__ dE, int64_t n,
-                                                   int64_t chunk, const double *__restrict__ stats,
-                                                   double *__restrict__ part) {
-    __shared__ double s_red[2 * 16];
+// Two-pass statistics of (z, dE): means, compute_std (utils.jl:531-544) and
+// Statistics.cor (the emittance figure of merit, figures_of_merit.jl:36-47).
+// Block b sums [b*chunk, (b+1)*chunk); mode 0 sums the values, mode 1 the
+// squared deviations and the cross product about stats[0..1].
+__global__ __launch_bounds__(1024) void k_pair_sums(int mode, const double *__restrict__ z,
+                                                    const double *__restrict__ dE, int64_t n,
+                                                    int64_t chunk, const double *__restrict__ stats,
+                                                    double *__restrict__ part) {
+    __shared__ double s_red[3 * 16];
     const int64_t i0 = (int64_t)blockIdx.x * chunk;
     const int64_t i1 = min(n, i0 + chunk);
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     const double cz = mode ? stats[0] : 0.0, ce = mode ? stats[1] : 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         if (mode) {
             const double a = z[i] - cz, b = dE[i] - ce;
             v[0] += a * a;
             v[1] += b * b;
+            v[2] += a * b;
         } else {
             v[0] += z[i];
             v[1] += dE[i];
         }
     }
-    block_sum<2>(v, s_red);
+    block_sum<3>(v, s_red);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = v[0];
-        part[2 * blockIdx.x + 1] = v[1];
+        part[3 * blockIdx.x] = v[0];
+        part[3 * blockIdx.x + 1] = v[1];
+        part[3 * blockIdx.x + 2] = v[2];
     }
 }
 
-// mode 0: stats[0..1] = means; mode 1: stats[2..3] = sample standard deviations
-__global__ __launch_bounds__(256) void k_gen_sums_final(int mode, const double *__restrict__ part,
-                                                        int nblocks, int64_t n,
-                                                        double *__restrict__ stats) {
-    __shared__ double s_red[2 * 16];
-    double v[2] = {0.0, 0.0};
+// mode 0: stats[0..1] = means; mode 1: stats[2..3] = sample standard
+// deviations of z and dE, stats[4] = their correlation
+__global__ __launch_bounds__(256) void k_pair_sums_final(int mode, const double *__restrict__ part,
+                                                         int nblocks, int64_t n,
+                                                         double *__restrict__ stats) {
+    __shared__ double s_red[3 * 16];
+    double v[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-        v[0] += part[2 * b];
-        v[1] += part[2 * b + 1];
+        v[0] += part[3 * b];
+        v[1] += part[3 * b + 1];
+        v[2] += part[3 * b + 2];
     }
-    block_sum<2>(v, s_red);
+    block_sum<3>(v, s_red);
     if (threadIdx.x == 0) {
         if (mode == 0) {
             stats[0] = v[0] / (double)n;
@@ -1216,6 +1221,8 @@ __global__ __launch_bounds__(256) void k_gen_sums_final(int mode, const double *
             const double vz = v[0] / (double)(n - 1), ve = v[1] / (double)(n - 1);
             stats[2] = sqrt(vz > 0.0 ? vz : 0.0);
             stats[3] = sqrt(ve > 0.0 ? ve : 0.0);
+            double r = v[2] / sqrt(v[0] * v[1]);   // Statistics.cor, clamped to [-1, 1]
+            stats[4] = (r != r) ? r : (r > 1.0 ? 1.0 : (r < -1.0 ? -1.0 : r));
         }
     }
 }
@@ -1387,18 +1394,24 @@ hipError_t launch_generate_particles(uint64_t seed, double mu_z, double mu_E, do
                                      double *z, double *dE, int64_t n, double *work,
                                      hipStream_t s) {
     if (n < 2) return hipSuccess;
-    double *L = work, *stats = work + 4, *part = work + 8;
+    double *L = work;
     hipLaunchKernelGGL(k_gen_cov, dim3(1), dim3(1024), 0, s, seed, mu_z, mu_E, sz, sE,
                        n < 10000 ? n : (int64_t)10000, L);
     hipLaunchKernelGGL(k_gen_mvn, dim3(blocks_for(n, 256)), dim3(256), 0, s, seed, mu_z, mu_E,
                        (const double *)L, z, dE, n);
+    return launch_pair_stats(z, dE, n, work + 4, work + 12, s);
+}
+
+hipError_t launch_pair_stats(const double *z, const double *dE, int64_t n, double *stats,
+                             double *part, hipStream_t s) {
+    if (n < 2) return hipSuccess;
     const int64_t nbl = (n + 1023) / 1024;
-    const int nb = (int)(nbl < kGenSumBlocks ? nbl : kGenSumBlocks);
+    const int nb = (int)(nbl < kStatBlocks ? nbl : kStatBlocks);
     const int64_t chunk = (n + nb - 1) / nb;
     for (int mode = 0; mode < 2; ++mode) {
-        hipLaunchKernelGGL(k_gen_sums, dim3(nb), dim3(1024), 0, s, mode, (const double *)z,
-                           (const double *)dE, n, chunk, (const double *)stats, part);
-        hipLaunchKernelGGL(k_gen_sums_final, dim3(1), dim3(256), 0, s, mode, (const double *)part,
+        hipLaunchKernelGGL(k_pair_sums, dim3(nb), dim3(1024), 0, s, mode, z, dE, n, chunk,
+                           (const double *)stats, part);
+        hipLaunchKernelGGL(k_pair_sums_final, dim3(1), dim3(256), 0, s, mode, (const double *)part,
                            nb, n, stats);
     }
     return hipGetLastError();

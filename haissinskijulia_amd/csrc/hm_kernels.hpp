@@ -66,11 +66,15 @@ hipError_t launch_generate(uint64_t seed, double mu_z, double sz, double mu_E, d
                            double *z, double *dE, int64_t n, hipStream_t s);
 hipError_t launch_turn_scalars(const BunchDev *bd_dev, double E0, TurnScalars *out,
                                hipStream_t s);
-// generate_particles for n >= 2 (sample-covariance MvNormal): writes z, dE and
-// work[0..2] = Cholesky factor (l11, l21, l22), work[4..7] = (mean z, mean dE,
-// std z, std dE); work holds kGenWorkDoubles doubles
-constexpr int kGenSumBlocks = 1024;
-constexpr int kGenWorkDoubles = 8 + 2 * kGenSumBlocks;
+// two-pass statistics of device arrays z, dE (n >= 2): stats[0..4] = (mean z,
+// mean dE, std z, std dE, cor(z, dE)); part holds 3 * kStatBlocks doubles
+constexpr int kStatBlocks = 1024;
+hipError_t launch_pair_stats(const double *z, const double *dE, int64_t n, double *stats,
+                             double *part, hipStream_t s);
+// generate_particles for n >= 2 (sample-covariance MvNormal): writes z, dE,
+// work[0..2] = Cholesky factor (l11, l21, l22) and the pair statistics of
+// the bunch at work[4..8]; work holds kGenWorkDoubles doubles
+constexpr int kGenWorkDoubles = 12 + 3 * kStatBlocks;
 hipError_t launch_generate_particles(uint64_t seed, double mu_z, double mu_E, double sz, double sE,
                                      double *z, double *dE, int64_t n, double *work,
                                      hipStream_t s);

@@ -230,6 +230,27 @@ hm_status hm_interpolate_kick(hm_ctx *ctx, const double *z, double *dE, int64_t 
     return stage_out(ctx, 1, dE, n);
 }
 
+hm_status hm_pair_stats(hm_ctx *ctx, const double *z, const double *dE, int64_t n, double *out) {
+    if (!ctx || n < 0 || (n && (!z || !dE)) || !out) return fail(HM_ERR_ARG, "bad arguments");
+    for (int k = 0; k < 5; ++k) out[k] = 0.0;
+    if (n == 1) {
+        out[0] = z[0];
+        out[1] = dE[0];
+    }
+    if (n < 2) return HM_OK;
+    Lock lk(ctx->mu);
+    HM_TRY(set_device(ctx));
+    double *dz, *de;
+    HM_TRY(stage_in(ctx, 0, z, n, &dz));
+    HM_TRY(stage_in(ctx, 1, dE, n, &de));
+    HM_TRY(ctx->sbuf[2].ensure((size_t)(8 + 3 * kStatBlocks)));
+    double *work = ctx->sbuf[2].p;
+    HIP_TRY(launch_pair_stats(dz, de, n, work, work + 8, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, work, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return HM_OK;
+}
+
 hm_status hm_global_std(hm_ctx *ctx, const double *x, int64_t n, double *out) {
     if (!ctx || n < 0 || (n && !x) || !out) return fail(HM_ERR_ARG, "bad arguments");
     if (n == 0) {

@@ -33,6 +33,7 @@ struct HmResult
     sigma_E::Float64; sigma_z::Float64; E0::Float64; turns_done::Int64
     drift_skipped::Int32; pad::Int32; ms_total::Float64; ms_pass_avg::Float64; ms_conv_avg::Float64
     pass_launches::Int64; sigma_E0::Float64; sigma_z0::Float64; bin_half_width::Float64
+    mean_z::Float64; mean_dE::Float64; corr_z_dE::Float64
 end
 
 last_error() = unsafe_string(ccall((:hm_last_error, LIB), Cstring, ()))

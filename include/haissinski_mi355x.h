@@ -82,6 +82,9 @@ typedef struct {
     int64_t pass_launches;  /* number of turn-pass launches timed */
     double sigma_E0, sigma_z0;  /* initial spreads (evolution.jl:245-253) */
     double bin_half_width;      /* z_width actually used [m] */
+    double mean_z, mean_dE;     /* final centroid (two-pass means) */
+    double corr_z_dE;           /* Statistics.cor of the final z, dE: the emittance figure
+                                   of merit (parameter_sensitivity/figures_of_merit.jl:36-47) */
 } hm_result;
 
 typedef struct hm_ctx hm_ctx;       /* one GPU, one stream, cached work buffers */
@@ -162,6 +165,10 @@ hm_status hm_interpolate_kick(hm_ctx *ctx, const double *z, double *dE, int64_t 
                               const double *centers, int64_t nbins, const double *P);
 /* compute_global_std (src/utils.jl:569-606) */
 hm_status hm_global_std(hm_ctx *ctx, const double *x, int64_t n, double *out);
+/* two-pass statistics of a bunch: out[0..4] = mean z, mean dE, compute_std of
+ * z and dE (src/utils.jl:531-544), Statistics.cor(z, dE) as used by the
+ * emittance figure of merit (src/parameter_sensitivity/figures_of_merit.jl:36-47) */
+hm_status hm_pair_stats(hm_ctx *ctx, const double *z, const double *dE, int64_t n, double *out);
 /* the device QE normal stream: normal of particles [i0, i0+n) at a turn */
 hm_status hm_normals(hm_ctx *ctx, uint64_t seed, int64_t turn, int64_t i0, int64_t n, double *out);
 /* bin edges/centres of range(-zw, zw, length=nbins+1) (evolution.jl:270-274),

@@ -171,3 +171,54 @@ def test_budget_admits_one_job_at_a_time():
         assert a1 >= d0 - 1e-9          # the next job starts after the previous one ends
     assert all(order[i].status == 0 for i in ids)
     srv.close()
+
+
+def test_pair_stats_match_numpy():
+    rng = np.random.default_rng(3)
+    z = rng.normal(1e-4, 4e-3, 300_001)
+    e = 2e5 * (z / 4e-3) + rng.normal(-50.0, 3e6, z.size)
+    s = H.pair_stats(z, e)
+    assert s["mean_z"] == pytest.approx(z.mean(), rel=1e-12)
+    assert s["std_z"] == pytest.approx(z.std(ddof=1), rel=1e-12)
+    assert s["std_dE"] == pytest.approx(e.std(ddof=1), rel=1e-12)
+    assert s["cor"] == pytest.approx(np.corrcoef(z, e)[0, 1], rel=1e-11)
+
+
+def test_emittance_figure_of_merit():
+    """compute_fom (figures_of_merit.jl:36-47) from the resident run's own
+    correlation and from the downloaded particles agree with numpy."""
+    s = _specs()[0]
+    b = _bunch(s)
+    r = b.evolve(_params(30, s["V"]), s["nbins"], _opts(s))
+    z, e = b.download()
+    b.close()
+    ref = r.sigma_z * r.sigma_E * np.sqrt(1 - np.corrcoef(z, e)[0, 1] ** 2)
+    f1 = H.compute_fom(H.EmittanceFoM(), None, r)
+    f2 = H.compute_fom(H.EmittanceFoM(), H.Particles(z, e), (r.sigma_E, r.sigma_z, r.E0))
+    assert f1 == pytest.approx(ref, rel=1e-10) and f2 == pytest.approx(ref, rel=1e-10)
+    assert H.compute_fom(H.EnergySpreadFoM(), None, r) == r.sigma_E
+    assert H.compute_fom(H.BunchLengthFoM(), None, r) == r.sigma_z
+
+
+def test_served_voltage_pair_is_a_common_random_number_difference():
+    """A sensitivity request runs V (1 -+ 0.5%) with the same seeds; served, its
+    members equal the two runs done alone, and the FD derivative follows."""
+    s = _specs()[2]
+    srv = H.Server(budget_bytes=8 << 30, max_slots=8)
+    other = srv.submit(_job(_specs()[1], 25))
+    lo, hi = srv.submit_voltage_pair(_job(s, 25), 0.005)
+    res = {r.job_id: r for r in srv.run(turns_per_tick=5)}
+    srv.close()
+    assert res[other].status == 0
+    alone = []
+    for f in (1 - 0.005, 1 + 0.005):
+        b = _bunch(s)
+        p = _params(25, s["V"]).replace(voltage=s["V"] * f)
+        alone.append(b.evolve(p, s["nbins"], _opts(s)))
+        b.close()
+    for jid, ra in zip((lo, hi), alone):
+        assert (res[jid].result.sigma_E, res[jid].result.E0) == (ra.sigma_E, ra.E0)
+    for fom in (H.EnergySpreadFoM(), H.BunchLengthFoM(), H.EmittanceFoM()):
+        d = H.fd_sensitivity(fom, res[lo].result, res[hi].result, s["V"], 0.005)
+        da = H.fd_sensitivity(fom, alone[0], alone[1], s["V"], 0.005)
+        assert np.isfinite(d) and d == da
