@@ -49,6 +49,7 @@ class HmOptions(C.Structure):
         ("wake_table", _dp), ("delta_sigma", C.c_double), ("bin_half_width_sigmas", C.c_double),
         ("charge", C.c_double), ("n_resonators", C.c_int32), ("timing", C.c_int32),
         ("resonators", _dp), ("fft_block", C.c_int32), ("block_size", C.c_int32),
+        ("monitor", _dp),
     ]
 
 

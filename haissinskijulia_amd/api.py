@@ -156,6 +156,7 @@ class TrackingOptions:
     timing: bool = False
     fft_block: int = 0
     block_size: int = 0
+    monitor: bool = False                # record per-turn centroid, sigma_z, E0, binned count
 
     def _c(self, keep: list, n: int, n_turns: int, nbins: int) -> HmOptions:
         o = HmOptions()
@@ -184,6 +185,11 @@ class TrackingOptions:
         o.timing = int(bool(self.timing))
         o.fft_block = int(self.fft_block)
         o.block_size = int(self.block_size)
+        if self.monitor and n_turns > 0:
+            mon = np.zeros((n_turns, 4))
+            keep.append(mon)
+            o.monitor = dptr(mon)
+            o._monitor = mon
         return o
 
 
@@ -204,6 +210,9 @@ class EvolveResult:
     mean_z: float = 0.0
     mean_dE: float = 0.0
     corr_z_dE: float = 0.0    # Statistics.cor of the final bunch (two-pass, on the GPU)
+    # per turn (TrackingOptions.monitor): z centroid and sigma_z after the drift
+    # (wake on), E0 after the turn, binned count
+    monitor: Optional[np.ndarray] = None
 
     @classmethod
     def _from(cls, r: HmResult) -> "EvolveResult":
@@ -321,7 +330,9 @@ class DeviceBunch:
         o = options._c(keep, self.n, params.n_turns, nbins)
         r = HmResult()
         check(_lib.load().hm_bunch_evolve(self._h, nbins, C.byref(params._c()), C.byref(o), C.byref(r)))
-        return EvolveResult._from(r)
+        res = EvolveResult._from(r)
+        res.monitor = getattr(o, "_monitor", None)
+        return res
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -348,11 +359,14 @@ def batch_evolve(bunches: Sequence[DeviceBunch], nbins: Sequence[int],
     nbs = np.ascontiguousarray(nbins, dtype=np.int64)
     ps = (HmParams * nb)(*[p._c() for p in params])
     opts = options or [TrackingOptions() for _ in range(nb)]
-    os_ = (HmOptions * nb)(*[o._c(keep, b.n, p.n_turns, k)
-                             for o, b, p, k in zip(opts, bunches, params, nbins)])
+    cs = [o._c(keep, b.n, p.n_turns, k) for o, b, p, k in zip(opts, bunches, params, nbins)]
+    os_ = (HmOptions * nb)(*cs)
     rs = (HmResult * nb)()
     check(_lib.load().hm_batch_evolve(dev.ctx, nb, hs, iptr(nbs), ps, os_, rs))
-    return [EvolveResult._from(r) for r in rs]
+    out = [EvolveResult._from(r) for r in rs]
+    for res, c in zip(out, cs):
+        res.monitor = getattr(c, "_monitor", None)
+    return out
 
 
 @dataclass
@@ -488,6 +502,7 @@ def longitudinal_evolve(particles: Particles, params: SimulationParameters,
     check(_lib.load().hm_evolve(dev.ctx, dptr(z), dptr(dE), n, nbins, C.byref(params._c()),
                                 C.byref(o), C.byref(r)))
     longitudinal_evolve.last = EvolveResult._from(r)
+    longitudinal_evolve.last.monitor = getattr(o, "_monitor", None)
     return r.sigma_E, r.sigma_z, r.E0
 
 

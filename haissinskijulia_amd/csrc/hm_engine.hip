@@ -414,6 +414,11 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
                                    hipMemcpyHostToDevice, s));
             d.normals = w.normals.p;
         }
+        if (q.o.monitor && q.n_turns > 0) {   // copied to q.o.monitor by run_batch
+            HM_TRY(w.mon.ensure((size_t)q.n_turns * 4));
+            HIP_TRY(hipMemsetAsync(w.mon.p, 0, (size_t)q.n_turns * 4 * sizeof(double), s));
+            d.mon = w.mon.p;
+        }
         q.o.normals = nullptr;  // host memory of the caller: not kept
         q.o.wake_table = nullptr;
         q.o.resonators = nullptr;
@@ -753,6 +758,10 @@ static hm_status run_batch(hm_ctx *ctx, int32_t nb, hm_bunch *const *bunches, co
     for (size_t k = 0; k < idx.size(); ++k) {
         HM_TRY(B.result(idx[k], &results[orig[k]], nullptr, nullptr));
         results[orig[k]].ms_total = total_ms;
+        const RunBunch &q = specs[k];
+        if (q.o.monitor && q.p.n_turns > 0)
+            HIP_TRY(hipMemcpy(q.o.monitor, q.b->w.mon.p, (size_t)q.p.n_turns * 4 * sizeof(double),
+                              hipMemcpyDeviceToHost));
     }
     return HM_OK;
 }

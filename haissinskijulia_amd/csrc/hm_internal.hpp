@@ -60,17 +60,17 @@ struct DevBuf {
 // per-bunch work buffers of an evolve call (kept between calls)
 struct Work {
     DevBuf<uint32_t> hist;
-    DevBuf<double> P, P2, lam, What, X, Wtmp, part, ipart, moments, normals, out, stats;
+    DevBuf<double> P, P2, lam, What, X, Wtmp, part, ipart, moments, normals, out, stats, mon;
     DevBuf<TurnScalars> sc;
     void release() {
         hist.release(); P.release(); P2.release(); lam.release(); What.release(); X.release();
         Wtmp.release(); part.release(); ipart.release(); moments.release(); normals.release();
-        out.release(); stats.release(); sc.release();
+        out.release(); stats.release(); mon.release(); sc.release();
     }
     size_t bytes() const {
         return hist.bytes() + P.bytes() + P2.bytes() + lam.bytes() + What.bytes() + X.bytes() +
                Wtmp.bytes() + part.bytes() + ipart.bytes() + moments.bytes() + normals.bytes() +
-               out.bytes() + stats.bytes() + sc.bytes();
+               out.bytes() + stats.bytes() + mon.bytes() + sc.bytes();
     }
 };
 

@@ -588,11 +588,12 @@ __global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
 struct BinScalars {
     double sz, cur, nh;
     int centre;   // interpolation cell of the centroid
+    double mz;    // centroid
 };
 
 // sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386), the
-// binned count and the centroid's cell from the tile sums; valid in every
-// thread.  s_bc holds 4 doubles.
+// binned count, the centroid and its cell from the tile sums; valid in every
+// thread.  s_bc holds 5 doubles.
 __device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, double *s_bc) {
     const TurnScalars &sc = *B.sc;
     double v[3] = {0.0, 0.0, 0.0};
@@ -613,9 +614,10 @@ __device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, doubl
         const double kc = floor((v[0] / (double)B.n - B.c_first) * B.inv_s);
         const double top = (double)(B.nbins - 1);
         s_bc[3] = (kc == kc) ? (kc < 0.0 ? 0.0 : (kc > top ? top : kc)) : 0.0;
+        s_bc[4] = v[0] / (double)B.n;
     }
     __syncthreads();
-    return BinScalars{s_bc[0], s_bc[1], s_bc[2], (int)s_bc[3]};
+    return BinScalars{s_bc[0], s_bc[1], s_bc[2], (int)s_bc[3], s_bc[4]};
 }
 
 __device__ __forceinline__ double line_density(const BunchDev &B, const uint32_t *h, int i,
@@ -681,7 +683,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fused(const BunchDev *__r
                                                              const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
     __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[4];
+    __shared__ double s_bc[5];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int o = items[2 * blockIdx.x + 1];
     const BinScalars bs = bin_scalars(B, s_red, s_bc);
@@ -690,6 +692,12 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fused(const BunchDev *__r
         B.sc->current = bs.cur;
         B.sc->n_h = bs.nh;
         B.sc->centre_bin = bs.centre;
+        if (B.mon) {   // turn monitor: centroid and sigma_z after the drift, binned count
+            double *m = B.mon + 4 * B.sc->turn;
+            m[0] = bs.mz;
+            m[1] = bs.sz;
+            m[3] = bs.nh;
+        }
     }
     clear_next_hist(B, o);
     if (bs.nh == 0.0 || B.nJ == 0) {   // wakefield.jl:305-311
@@ -710,7 +718,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
                                                            const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
     __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[4];
+    __shared__ double s_bc[5];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int a = items[2 * blockIdx.x + 1];
     const BinScalars bs = bin_scalars(B, s_red, s_bc);
@@ -719,6 +727,12 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
         B.sc->current = bs.cur;
         B.sc->n_h = bs.nh;
         B.sc->centre_bin = bs.centre;
+        if (B.mon) {   // turn monitor: centroid and sigma_z after the drift, binned count
+            double *m = B.mon + 4 * B.sc->turn;
+            m[0] = bs.mz;
+            m[1] = bs.sz;
+            m[3] = bs.nh;
+        }
     }
     clear_next_hist(B, a);
     if (bs.nh == 0.0) return;
@@ -734,7 +748,7 @@ __global__ __launch_bounds__(kConvInvThreads) void k_conv_inv(const BunchDev *__
                                                               const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
     __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[4];
+    __shared__ double s_bc[5];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int o = items[2 * blockIdx.x + 1];
     const BinScalars bs = bin_scalars(B, s_red, s_bc);
@@ -978,6 +992,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__re
             s.pending_m = 0;
         }
         s.pending_kick = (fl & F_WAKE) ? 1 : 0;
+        if (B.mon) B.mon[4 * s.turn + 2] = E0;   // turn monitor: E0 after the turn
         s.turn += 1;
         turn_scalars(B, E0, s);
         *B.sc = s;

@@ -169,6 +169,7 @@ hm_status hm_server_destroy(hm_server *s) {
 hm_status hm_server_submit(hm_server *s, const hm_job *job, int64_t *job_id) {
     if (!s || !job) return fail(HM_ERR_ARG, "bad arguments");
     if (job->opts.normals) return fail(HM_ERR_ARG, "served jobs draw their own normals");
+    if (job->opts.monitor) return fail(HM_ERR_ARG, "served jobs keep no turn monitor");
     HM_TRY(validate(&job->params, &job->opts, job->n, job->nbins));
     if (job->n == 0) return fail(HM_ERR_ARG, "served jobs need particles");
     std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);

@@ -93,6 +93,7 @@ struct BunchDev {
     int32_t win;          // entries of P the pass stages in LDS (0: all gathers global)
     double *part, *ipart, *moments, *out;
     TurnScalars *sc;
+    double *mon;          // device [n_turns][4] turn monitor or null
 };
 
 // LDS of a turn-pass workgroup for one bunch: its histogram (nbins counts,

@@ -66,6 +66,11 @@ typedef struct {
     const double *resonators;   /* host [n_resonators][3] = (R_s [Ohm], f_r [Hz], Q) */
     int32_t fft_block;          /* 0 = automatic; else the convolution block size (power of 2) */
     int32_t block_size;         /* 0 = automatic; else threads per block of the turn pass (1024) */
+    double *monitor;            /* host [n_turns][4] or NULL: per turn the z centroid and sigma_z
+                                   after the drift (wake on; 0 otherwise), E0 after the turn and
+                                   the binned count -- the moments of the turn-by-turn states of
+                                   BeamTurn / assign_to_turn! (data_structures.jl:52-54,
+                                   utils.jl:42-50) */
 } hm_options;
 
 /* Return of longitudinal_evolve! (evolution.jl:470) plus diagnostics. */

@@ -663,6 +663,12 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
 
         if (p->use_wakefield) {
             double sz = gstd_ranks(z, ch, R, scratch);
+            if (o->monitor) {
+                double s = 0.0;
+                for (int64_t i = 0; i < n; ++i) s += z[i];
+                o->monitor[4 * t] = s / (double)n;
+                o->monitor[4 * t + 1] = sz;
+            }
             double gw = E0 / mass;
             double ew = alpha_c - (1.0 / (gw * gw));
             double current = o->charge > 0 ? o->charge
@@ -675,6 +681,11 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
                 int64_t s = 0;
                 for (int r = 0; r < R; ++r) s += rcounts[(size_t)r * (size_t)nbins + b];
                 counts[b] = s;
+            }
+            if (o->monitor) {
+                int64_t nh = 0;
+                for (int64_t b = 0; b < nbins; ++b) nh += counts[b];
+                o->monitor[4 * t + 3] = (double)nh;
             }
             or_wake_potential(counts, nbins, wtab, ltab, current, P);
 #pragma omp parallel for schedule(static, 1) num_threads(R)
@@ -700,6 +711,7 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
             if (n > 0)
                 for (int64_t i = 0; i < n; ++i) dE[i] -= mean;
         }
+        if (o->monitor) o->monitor[4 * t + 2] = E0;
     }
 
     res->sigma_E = gstd_ranks(dE, ch, R, scratch);

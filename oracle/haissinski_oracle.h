@@ -45,6 +45,8 @@ typedef struct {
     int32_t n_resonators;          /* >0: W by lag = sum of broadband resonators at s = j*step */
     int32_t pad2_;
     const double *resonators;      /* [n_resonators][3] = (R_s [Ohm], f_r [Hz], Q) */
+    double *monitor;               /* [n_turns][4] or NULL: z centroid and sigma_z after the drift
+                                      (wake on), E0 after the turn, binned count */
 } or_options;
 
 typedef struct {

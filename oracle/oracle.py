@@ -41,7 +41,7 @@ class OrOptions(C.Structure):
         ("nranks", C.c_int32), ("pad_", C.c_int32), ("wake_table", _dp),
         ("delta_sigma", C.c_double), ("bin_half_width_sigmas", C.c_double),
         ("charge", C.c_double), ("n_resonators", C.c_int32), ("pad2_", C.c_int32),
-        ("resonators", _dp),
+        ("resonators", _dp), ("monitor", _dp),
     ]
 
 
@@ -275,7 +275,7 @@ class EvolveResult:
 
 def evolve(z, dE, nbins, params: dict, normals=None, seed=0, turn_offset=0, nranks=1,
            wake_table=None, delta_sigma=0.0, bin_half_width_sigmas=0.0, charge=0.0,
-           resonators=None):
+           resonators=None, monitor=None):
     """Run the restated turn loop in place on copies; returns (z, dE, EvolveResult)."""
     z = f64(z).copy(); dE = f64(dE).copy()
     p = OrParams(**{k: params[k] for k, _ in OrParams._fields_})
@@ -301,6 +301,9 @@ def evolve(z, dE, nbins, params: dict, normals=None, seed=0, turn_offset=0, nran
         keep.append(rs)
         o.n_resonators = rs.size // 3
         o.resonators = _d(rs)
+    if monitor is not None:   # float64 [n_turns, 4], filled in place
+        assert monitor.dtype == np.float64 and monitor.size >= 4 * params["n_turns"]
+        o.monitor = _d(monitor)
     res = OrResult()
     P = np.zeros(max(nbins, 1))
     st = lib().or_evolve(_d(z), _d(dE), z.size, nbins, C.byref(p), C.byref(o), C.byref(res), _d(P))

@@ -28,7 +28,7 @@ def test_struct_layouts_match_header_sizes():
     # hm_params: 3 doubles + int64 + 5 doubles + int64 + 5 int32 (+pad)
     assert C.sizeof(_lib.HmParams) == 8 * 10 + 4 * 5 + 4
     assert C.sizeof(_lib.HmResult) == 8 * 3 + 8 + 4 + 4 + 8 * 3 + 8 + 8 * 3 + 8 * 3
-    assert C.sizeof(_lib.HmOptions) == 8 + 8 + 8 + 8 + 8 * 3 + 4 + 4 + 8 + 4 + 4
+    assert C.sizeof(_lib.HmOptions) == 8 + 8 + 8 + 8 + 8 * 3 + 4 + 4 + 8 + 4 + 4 + 8
 
 
 @pytest.mark.parametrize("zw,nb", [(0.03, 1024), (7.5 * 4.123456789e-3, 1000), (1e-6, 7),

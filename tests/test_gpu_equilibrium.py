@@ -79,3 +79,39 @@ def test_equilibrium_moments_and_profile_match_oracle_ensembles():
     lim = float(stats.chi2.ppf(0.999, keep.sum()))
     print(f"profile chi2 = {chi2:.1f} over {keep.sum()} bins (0.1% limit {lim:.1f})", flush=True)
     assert chi2 <= lim
+
+
+def test_equilibrium_matches_haissinski_solution():
+    """The GPU path's long-time state against the numerical Haissinski
+    solution (tests/haissinski_solver.py) on the 3 GeV ring with the broadband
+    resonator at 1 nC, update_E0 and update_eta off: 1e6 particles, 20000 turns
+    in one call; centroid and sigma_z averaged over the second half from the
+    turn monitor agree with the solver within 1% and 0.5% of sigma_z, the final
+    skewness within 4 standard errors plus 0.01, sigma_E with the excitation
+    level sigma_E0."""
+    import math
+    from haissinski_solver import moments, sample_moments, solve
+    n, nb, turns, charge = 1_000_000, 512, 20_000, 1e-9
+    p = W.ring_3gev_lattice(turns, flags=(True, False, False, True, True))
+    c, lam, _ = solve(p, nb, 0.03, [W.RESONATOR_BB], charge, 3e6)
+    mu0, sd0, _ = moments(c, lam)
+    b = H.DeviceBunch(n)
+    b.generate(7, sd0, 3e6, mu_z=mu0)
+    r = b.evolve(H.SimulationParameters.from_dict(p), nb,
+                 H.TrackingOptions(seed=8, resonators=[W.RESONATOR_BB], charge=charge,
+                                   bin_half_width_sigmas=8.0, monitor=True))
+    z, e = b.download()
+    b.close()
+    mon = r.monitor
+    assert mon.shape == (turns, 4) and np.all(mon[:, 3] == n) and np.all(mon[:, 2] == p["E0"])
+    c, lam, _ = solve(p, nb, 0.03, [W.RESONATOR_BB], charge, r.sigma_E0)
+    mu, sd, sk = moments(c, lam)
+    late = mon[turns // 2:]
+    zm, zs = float(late[:, 0].mean()), float(late[:, 1].mean())
+    zk = sample_moments(z)[2]
+    print(f"solver mu {mu:.5e} sd {sd:.5e} skew {sk:.4f}; gpu mu {zm:.5e} sd {zs:.5e} "
+          f"skew {zk:.4f}; sigma_E {r.sigma_E:.5e} vs sigma_E0 {r.sigma_E0:.5e}", flush=True)
+    assert abs(zm - mu) <= 0.01 * sd
+    assert abs(zs - sd) <= 0.005 * sd
+    assert abs(zk - sk) <= 4 * math.sqrt(6 / n) + 0.01
+    assert abs(r.sigma_E - r.sigma_E0) <= 0.005 * r.sigma_E0

@@ -1,0 +1,69 @@
+"""The oracle's equilibrium against a numerical Haissinski solution.
+
+Pins the long-time behaviour of or_evolve (RF + wake + SR + QE; update_E0 and
+update_eta off) that the turn-by-turn checks leave open: a 2e4-particle
+bunch tracked for 20000 turns (about five damping times) on the 3 GeV ring
+with the broadband resonator at 1 nC settles to the self-consistent
+Haissinski profile -- shifted (the wake's energy loss moves the synchronous
+phase), lengthened and skewed.  Its centroid and length, averaged over the
+second half of the run, must agree with the solver within 1% and 0.5% of the
+bunch length (the smooth-motion approximation and the residual coherent
+motion), its skewness within 4 standard errors plus 0.01.
+"""
+import math
+
+import numpy as np
+
+from haissinski_solver import moments, sample_moments, solve
+from helpers import W
+from oracle import oracle as O
+
+N, NB, HW, CHARGE = 20_000, 512, 8.0, 1e-9
+TURNS = 20_000   # about five damping times
+
+
+def test_solver_without_wake_is_the_linear_bucket():
+    p = W.ring_3gev_lattice(1)
+    c, lam, P = solve(p, NB, 0.03, [W.RESONATOR_BB], 0.0, 3e6, iters=50)
+    mu, sd, sk = moments(c, lam)
+    g0 = p["E0"] / p["mass"]
+    b0 = math.sqrt(1 - 1 / g0 ** 2)
+    eta0 = p["alpha_c"] - 1 / g0 ** 2
+    rf = p["freq_rf"] * 2 * math.pi / (b0 * 299792458.0)
+    K = (2 * math.pi * p["harmonic"] * eta0 / (b0 * b0 * p["E0"])) / rf
+    a = abs(p["voltage"] * math.cos(p["phi_s"]) * rf)
+    # small-amplitude bucket: sigma_z = sigma_E sqrt(K / |V cos(phi_s) rf|); the
+    # RF curvature shifts and skews it at the 1% level
+    np.testing.assert_allclose(sd, 3e6 * math.sqrt(K / a), rtol=0.02)
+    assert abs(mu) < 0.02 * sd and abs(sk) < 0.05
+
+
+def test_oracle_equilibrium_matches_haissinski_solution():
+    """One call of TURNS turns, so that the excitation level sigma_E0 (measured
+    at the start of the call) sets the equilibrium energy spread throughout.
+    The centroid and sigma_z are averaged over the second half of the run from
+    the turn monitor (a single snapshot carries a slowly decaying coherent
+    centroid oscillation of a few percent of sigma_z)."""
+    p = W.ring_3gev_lattice(TURNS, flags=(True, False, False, True, True))
+    # start near the equilibrium, so that no large coherent oscillation remains
+    c, lam, _ = solve(p, NB, 0.03, [W.RESONATOR_BB], CHARGE, 3e6)
+    mu0, sd0, _ = moments(c, lam)
+    z0, e0 = O.generate_gaussian(3, mu0, sd0, 0.0, 3e6, N)
+    sE0 = O.global_std(e0)
+    mon = np.zeros((TURNS, 4))
+    z, e, r = O.evolve(z0, e0, NB, p, seed=100, nranks=8, resonators=[W.RESONATOR_BB],
+                       charge=CHARGE, bin_half_width_sigmas=HW, monitor=mon)
+    assert r.status == 0
+    assert np.all(mon[:, 3] == N) and np.all(mon[:, 2] == p["E0"])   # all binned; E0 fixed
+    c, lam, _ = solve(p, NB, 0.03, [W.RESONATOR_BB], CHARGE, sE0)
+    mu, sd, sk = moments(c, lam)
+    late = mon[TURNS // 2:]
+    zm, zs = float(late[:, 0].mean()), float(late[:, 1].mean())
+    zk = sample_moments(z)[2]
+    print(f"solver mu {mu:.5e} sd {sd:.5e} skew {sk:.4f}; oracle mu {zm:.5e} sd {zs:.5e} "
+          f"skew {zk:.4f}; sigma_E {np.std(e):.5e} vs sigma_E0 {sE0:.5e}", flush=True)
+    assert abs(zm - mu) <= 0.01 * sd
+    assert abs(zs - sd) <= 0.005 * sd
+    assert abs(zk - sk) <= 4 * math.sqrt(6 / N) + 0.01
+    assert abs(np.std(e) - sE0) <= 4 * sE0 / math.sqrt(2 * N) + 0.005 * sE0
+    assert abs(mu) > 0.5 * sd and sk < -0.2      # the wake visibly shifts and skews the bunch
