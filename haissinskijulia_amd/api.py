@@ -33,6 +33,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
+from . import tune as _tune
 from ._lib import HaissinskiError, HmOptions, HmParams, HmResult, check, dptr, iptr
 
 SPEED_LIGHT = 299792458.0          # src/constants.jl:5
@@ -183,7 +184,8 @@ class TrackingOptions:
         o.bin_half_width_sigmas = self.bin_half_width_sigmas
         o.charge = self.charge
         o.timing = int(bool(self.timing))
-        o.fft_block = int(self.fft_block)
+        # 0: the tuned block for this bin count (tuning.json), else the library's rule
+        o.fft_block = int(self.fft_block) or _tune.fft_block_for(nbins)
         o.block_size = int(self.block_size)
         if self.monitor and n_turns > 0:
             mon = np.zeros((n_turns, 4))
