@@ -213,10 +213,12 @@ hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t n
 static int64_t pairs_per_tile_for(int64_t npairs, int threads) {
     // tiling depends on the bunch size only, so a bunch is reduced in the same
     // order alone and in any batch
+    // (about 1024 tiles of a large bunch, four per CU; at least four pairs per
+    // thread, so that a tile's fixed costs stay small for mid-size bunches)
     const int64_t target_tiles = 1024;
     int64_t ppt = (npairs + target_tiles - 1) / target_tiles;
     ppt = ((ppt + threads - 1) / threads) * threads;
-    return std::max<int64_t>(ppt, threads);
+    return std::max<int64_t>(ppt, 4 * (int64_t)threads);
 }
 
 Batch::~Batch() {
@@ -252,6 +254,7 @@ hm_status Batch::build_tables() {
     Fmax_ = 2;
     any_wake_ = any_wake_e0_ = false;
     std_only_ = true;
+    interp_lds_ = 0;
     for (size_t i = 0; i < rb_.size(); ++i) {
         RunBunch &q = rb_[i];
         if (!used_[i] || !q.active) continue;
@@ -261,7 +264,10 @@ hm_status Batch::build_tables() {
         if ((q.d.flags & kPhysMask) != kStdFlags) std_only_ = false;
         if (q.d.flags & F_WAKE) {
             any_wake_ = true;
-            if (q.d.flags & F_E0) any_wake_e0_ = true;
+            if (q.d.flags & F_E0) {
+                any_wake_e0_ = true;
+                interp_lds_ = std::max(interp_lds_, (size_t)q.d.iwin * 8);
+            }
             for (int o = 0; o < q.d.nA; ++o) {
                 if (q.d.nJ <= 1) {
                     conv[CONV_FUSED].push_back((int32_t)i);
@@ -396,8 +402,8 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         d.turn_offset = q.o.turn_offset;
         d.nbins = (int32_t)(p->use_wakefield ? q.nbins : 0);
         Work &w = b->w;
-        HM_TRY(w.part.ensure((size_t)std::max(d.ntiles, 1) * 4));
-        HM_TRY(w.ipart.ensure((size_t)std::max(d.ntiles, 1)));
+        HM_TRY(w.part.ensure((size_t)std::max(d.ntiles, 1) * kTileWaves * 4));
+        HM_TRY(w.ipart.ensure((size_t)std::max(d.ntiles, 1) * kTileWaves));
         HM_TRY(w.moments.ensure((size_t)std::max(d.ntiles, 1) * 4));
         HM_TRY(w.sc.ensure(1));
         HM_TRY(w.out.ensure(16));
@@ -498,6 +504,8 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         {   // the pass's LDS window of P: what the histogram leaves of the LDS
             const int64_t cap = ((int64_t)ctx_->lds_max - 1024 - (int64_t)pass_hist_bytes(d)) / 8;
             d.win = cap >= 2 ? (int32_t)std::min<int64_t>(nb, cap) : 0;
+            // the interpolation sums' window: the whole table when it fits
+            d.iwin = (int32_t)std::min<int64_t>(nb, ((int64_t)ctx_->lds_max - 1024) / 8);
         }
         Work &w = q.b->w;
         HM_TRY(w.hist.ensure((size_t)nb * 2));
@@ -571,8 +579,11 @@ hm_status Batch::one_turn() {
         HIP_TRY(launch_conv(CONV_INV, d_bd_.p, d_conv_[CONV_INV].p, nconv_[CONV_INV], Fmax_, s));
     }
     if (timing_) HIP_TRY(hipEventRecord(e2, s));
-    if (any_wake_e0_)   // two 64 KiB-LDS workgroups per CU
-        HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, ntiles_, std::min(ntiles_, 2 * ctx_->cus), s));
+    if (any_wake_e0_) {   // two workgroups per CU when both windows fit the LDS
+        const int per_cu = interp_lds_ + 1024 <= (size_t)ctx_->lds_max / 2 ? 2 : 1;
+        HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, ntiles_,
+                                  std::min(ntiles_, per_cu * ctx_->cus), interp_lds_, s));
+    }
     HIP_TRY(launch_finalize(d_bd_.p, d_list_.p, nlist_, s));
     return HM_OK;
 }

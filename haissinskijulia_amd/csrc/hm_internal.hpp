@@ -175,6 +175,7 @@ class Batch {
     int Fmax_ = 2;
     bool any_wake_ = false, any_wake_e0_ = false;   // some bunch with wake / wake + update_E0
     bool std_only_ = true;                          // all active bunches have kStdFlags
+    size_t interp_lds_ = 0;                         // dynamic LDS of the interpolation sums
     bool dirty_ = true;
     bool timing_ = false;
     std::vector<hipEvent_t> ev_pool_;

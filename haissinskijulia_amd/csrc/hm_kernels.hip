@@ -209,6 +209,16 @@ __device__ inline void block_sum(double (&v)[NV], double *s_red) {
     }
 }
 
+// Butterfly sum of NV values over one wave; every lane gets the result.
+template <int NV>
+__device__ __forceinline__ void wave_sum(double (&v)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    }
+}
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -386,7 +396,7 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
 // evaluate the same expressions, so a bunch gets the same bits from either.
 // LH: histogram in LDS at offset 0; the window of P follows at byte hb.
 template <uint32_t FX, bool LH>
-__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, int hb, double *s_red) {
+__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, int hb) {
     PassCtx c;
     load_ctx(B, c);
     const Flags<FX> fl{c.flags};
@@ -495,12 +505,13 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, int hb
         ep[i] = e0;
     }
     if (!wake) { v[0] = 0.0; v[1] = 0.0; }
-    block_sum<4>(v, s_red);
-    if (threadIdx.x == 0) {
-        double *o = B.part + 4 * (int64_t)lt;
+    // per-wave partial sums: no workgroup barrier per tile, so a wave never
+    // waits for the others of its workgroup between tiles
+    wave_sum<4>(v);
+    if ((threadIdx.x & 63) == 0) {
+        double *o = B.part + 4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6));
         o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
     }
-    __syncthreads();   // s_red is reused by the next tile
 }
 
 // Adds the LDS histogram of bunch B into this turn's global counts.
@@ -538,7 +549,6 @@ template <uint32_t FX>
 __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bds,
                                                  const int32_t *__restrict__ tile_bunch,
                                                  int ntiles) {
-    __shared__ double s_red[4 * 16];
     // kStdFlags has the wake on and F_GHIST off: its histogram is in LDS
     constexpr bool kLdsOnly = (FX & F_WAKE) && !(FX & F_GHIST);
     const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
@@ -555,9 +565,9 @@ __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bd
         }
         const int hb = pass_hist_bytes(B);
         if (kLdsOnly || hb)
-            turn_pass_tile<FX, true>(B, tile - B.tile_begin, hb, s_red);
+            turn_pass_tile<FX, true>(B, tile - B.tile_begin, hb);
         else
-            turn_pass_tile<FX, false>(B, tile - B.tile_begin, hb, s_red);
+            turn_pass_tile<FX, false>(B, tile - B.tile_begin, hb);
     }
     if (staged >= 0) flush_hist(bds[staged]);
 }
@@ -597,7 +607,7 @@ struct BinScalars {
 __device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, double *s_bc) {
     const TurnScalars &sc = *B.sc;
     double v[3] = {0.0, 0.0, 0.0};
-    for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
+    for (int t = threadIdx.x; t < B.ntiles * kTileWaves; t += blockDim.x) {
         v[0] += B.part[4 * t];
         v[1] += B.part[4 * t + 1];
         v[2] += B.part[4 * t + 3];
@@ -808,14 +818,13 @@ __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__res
 
 // Sum over a tile of the interpolated potential at the post-drift z (the
 // E0 update of a wake bunch).  Each workgroup walks a contiguous range of
-// tiles and stages, once per bunch it meets, a window of kInterpWindow
-// entries of P around the centroid in LDS; the few particles outside it read
-// P from global memory.  Each thread adds its pairs in ascending order, so
+// tiles and stages, once per bunch it meets, a window of B.iwin entries of P
+// around the centroid in LDS (the whole table unless it exceeds the LDS); the
+// particles outside it read P from global memory.  Each thread adds its pairs in ascending order, so
 // the per-tile sums depend on the tile and the block size only.
 __global__ __launch_bounds__(kInterpThreads) void k_interp_sum(
     const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch, int ntiles) {
-    __shared__ double sP[kInterpWindow];
-    __shared__ double s_red[16];
+    extern __shared__ double sP[];   // B.iwin entries of P
     const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per;
     const int t1 = min(ntiles, t0 + per);
@@ -828,7 +837,7 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_sum(
         const int nb = B.nbins;
         const double *__restrict__ P = B.P;
         if (bi != staged) {
-            const int w = min(nb, kInterpWindow);
+            const int w = B.iwin;
             wlo = window_lo(B.sc->centre_bin, w, nb);
             wpairs = (unsigned)(w - 1);
             __syncthreads();
@@ -878,9 +887,8 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_sum(
         }
         if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1)
             v[0] += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
-        block_sum<1>(v, s_red);
-        if (threadIdx.x == 0) B.ipart[lt] = v[0];
-        __syncthreads();   // s_red is reused by the next tile
+        wave_sum<1>(v);
+        if ((threadIdx.x & 63) == 0) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = v[0];
     }
 }
 
@@ -973,7 +981,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__re
     const uint32_t fl = B.flags;
     double v[2] = {0.0, 0.0};   // sum (dE'-dE_init), sum interpolated P
     if (fl & F_E0) {
-        for (int t = threadIdx.x; t < B.ntiles; t += blockDim.x) {
+        for (int t = threadIdx.x; t < B.ntiles * kTileWaves; t += blockDim.x) {
             v[0] += B.part[4 * t + 2];
             if (fl & F_WAKE) v[1] += B.ipart[t];
         }
@@ -1242,14 +1250,14 @@ __global__ __launch_bounds__(256) void k_pair_sums_final(int mode, const double 
     }
 }
 
-// Plain device copy (16 B per lane per step): the HBM ceiling of a kernel
-// that reads and writes each byte once, and the calibration of the HBM
-// byte counters.
-__global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ src,
-                                                     double2 *__restrict__ dst, int64_t n2) {
+// Plain device copy (16 B per lane per step, streaming loads and stores,
+// 64 workgroups of 1024 per CU): the HBM ceiling of a kernel that reads and
+// writes each byte once, and the calibration of the HBM byte counters.
+__global__ __launch_bounds__(1024) void k_stream_copy(const f64x2 *__restrict__ src,
+                                                      f64x2 *__restrict__ dst, int64_t n2) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
          i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1262,7 +1270,8 @@ hipError_t configure_kernels(int lds) {
     const void *fns[] = {(const void *)k_turn_pass, (const void *)k_turn_pass_std,
                          (const void *)k_conv_fused,
                          (const void *)k_conv_fwd, (const void *)k_conv_inv,
-                         (const void *)k_fft_wake, (const void *)k_stage_hist};
+                         (const void *)k_fft_wake, (const void *)k_stage_hist,
+                         (const void *)k_interp_sum};
     hipError_t last = hipSuccess;
     for (const void *f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -1273,8 +1282,8 @@ hipError_t configure_kernels(int lds) {
 
 hipError_t launch_stream_copy(const double *src, double *dst, int64_t n, int blocks, hipStream_t s) {
     if (n <= 1) return hipSuccess;
-    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s,
-                       reinterpret_cast<const double2 *>(src), reinterpret_cast<double2 *>(dst), n / 2);
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(1024), 0, s,
+                       reinterpret_cast<const f64x2 *>(src), reinterpret_cast<f64x2 *>(dst), n / 2);
     return hipGetLastError();
 }
 
@@ -1324,10 +1333,10 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
 }
 
 hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, int ntiles, int grid,
-                             hipStream_t s) {
+                             size_t lds_bytes, hipStream_t s) {
     if (ntiles <= 0 || grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_interp_sum, dim3((unsigned)grid), dim3(kInterpThreads), 0, s, bds, tb,
-                       ntiles);
+    hipLaunchKernelGGL(k_interp_sum, dim3((unsigned)grid), dim3(kInterpThreads), lds_bytes, s, bds,
+                       tb, ntiles);
     return hipGetLastError();
 }
 

@@ -14,7 +14,11 @@ constexpr int kConvThreads = 256;      // convolution block
 constexpr int kConvInvThreads = 1024;  // phase-B convolution block (spectral sums in flight)
 constexpr int kFinalThreads = 256;     // per-bunch finalize block
 constexpr int kInterpThreads = 1024;   // LDS-staged interpolation sums
-constexpr int kInterpWindow = 8192;    // entries of P they stage (64 KiB: two workgroups per CU)
+// partial sums per tile are kept per wave of the 1024-thread pass and
+// interpolation-sum workgroups: part[ntiles][kTileWaves][4], ipart[ntiles][kTileWaves]
+constexpr int kTileWaves = 16;
+static_assert(kPassThreads == 64 * kTileWaves && kInterpThreads == 64 * kTileWaves,
+              "per-wave tile sums assume 1024-thread workgroups");
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (128 KiB of complex FP64)
 constexpr int kAutoFFTBlock = 1024;    // default convolution block (F = 2048)
 
@@ -40,9 +44,9 @@ hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, int 
 hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
                        hipStream_t s);
 // interpolation sums of the wake + update_E0 bunches: `grid` workgroups over
-// the tile table
+// the tile table; lds_bytes = 8 x the largest iwin among them
 hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int grid,
-                             hipStream_t s);
+                             size_t lds_bytes, hipStream_t s);
 hipError_t launch_finalize(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
 // FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
 hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,

@@ -167,13 +167,16 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     HM_TRY(ctx->sbuf[0].ensure((size_t)nbins));   // W
     HM_TRY(ctx->sbuf[1].ensure((size_t)nbins));   // lam
     HM_TRY(ctx->sbuf[2].ensure((size_t)nbins));   // P
-    HM_TRY(ctx->sbuf[3].ensure(what + xs + 8));   // What, X, tile partials
+    // What, X, and the tile partials of one tile (kTileWaves x 4: only the
+    // binned count n_h is non-zero, the bin phase reads them all)
+    HM_TRY(ctx->sbuf[3].ensure(what + xs + 4 * kTileWaves));
     HM_TRY(ctx->ssc.ensure(1));
     HM_TRY(ctx->sbd.ensure(1));
     double *What = ctx->sbuf[3].p;
     double *part = What + what + xs;
     d.X = What + what;
-    const double hp[4] = {0.0, 0.0, 0.0, (double)nh};
+    double hp[4 * kTileWaves] = {};
+    hp[3] = (double)nh;
     const double *tw;
     HM_TRY(twiddles(ctx, d.F, &tw));
     d.hist = ctx->shist.p;   // counts in the first half (turn 0); the kernels clear the second
@@ -282,7 +285,7 @@ hm_status hm_stream_copy(hm_ctx *ctx, int64_t bytes, int32_t iters, double *ms_a
     HM_TRY(b.ensure((size_t)n));
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemsetAsync(a.p, 0, (size_t)n * 8, s));
-    const int blocks = ctx->cus * 8;
+    const int blocks = ctx->cus * 64;   // the fastest shape of a sweep on MI355X
     HIP_TRY(launch_stream_copy(a.p, b.p, n, blocks, s));   // warm-up
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));

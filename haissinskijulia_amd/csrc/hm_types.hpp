@@ -91,6 +91,7 @@ struct BunchDev {
     double *X;            // interleaved complex [nA][F][2]: FFT(g_a + i g_{a-1}) (nJ > 1)
     int32_t B, F, logF, nA, nJ;
     int32_t win;          // entries of P the pass stages in LDS (0: all gathers global)
+    int32_t iwin;         // entries of P the interpolation sums stage in LDS
     double *part, *ipart, *moments, *out;
     TurnScalars *sc;
     double *mon;          // device [n_turns][4] turn monitor or null
