@@ -5,7 +5,7 @@
 //   set-up   k_moments + k_moments_final   sigma_E0, sigma_z0 (evolution.jl:245-253)
 //            host: bin grid, wake and line-density tables (evolution.jl:261-281,
 //            wakefield.jl:194-221), FFT block plan; k_fft_wake; k_init
-//   turns    k_turn_pass, k_conv, [k_interp_sum], k_finalize   per turn
+//   turns    k_turn_pass, k_bin_reduce, k_conv, [k_interp_sum], k_finalize   per turn
 //   epilogue k_moments(mode 1) + k_moments_final            (evolution.jl:461-470)
 // A batch of bunches shares every launch: tiles, convolution items and
 // finalize blocks are looked up through per-launch tables.
@@ -211,14 +211,34 @@ hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t n
 // ---------------------------------------------------------------------------
 
 static int64_t pairs_per_tile_for(int64_t npairs, int threads) {
-    // tiling depends on the bunch size only, so a bunch is reduced in the same
-    // order alone and in any batch
-    // (about 1024 tiles of a large bunch, four per CU; at least four pairs per
-    // thread, so that a tile's fixed costs stay small for mid-size bunches)
+    // tiling depends on the bunch size alone, so a bunch is reduced in the
+    // same order alone and in any batch: about 1024 tiles of a large bunch
+    // (four per CU), at least one pair per thread.  A workgroup streams its
+    // run of consecutive tiles of a bunch as one loop, so small tiles cost
+    // no extra pipeline start.
     const int64_t target_tiles = 1024;
     int64_t ppt = (npairs + target_tiles - 1) / target_tiles;
     ppt = ((ppt + threads - 1) / threads) * threads;
-    return std::max<int64_t>(ppt, 4 * (int64_t)threads);
+    return std::max<int64_t>(ppt, (int64_t)threads);
+}
+
+// Splits tiles with weights w (pairs) into `grid` contiguous ranges of
+// about equal weight; appends grid + 1 boundaries (none for grid = 0).
+static void balanced_ranges(const std::vector<int64_t> &w, int grid, std::vector<int32_t> &out) {
+    if (grid <= 0) return;
+    int64_t total = 0;
+    for (int64_t x : w) total += x;
+    const int nt = (int)w.size();
+    out.push_back(0);
+    int t = 0;
+    int64_t acc = 0;
+    for (int g = 1; g < grid; ++g) {
+        // the first tile at which the running weight reaches g/grid of the total
+        const int64_t goal = (int64_t)(((__int128)total * g) / grid);
+        while (t < nt && acc + w[(size_t)t] <= goal) acc += w[(size_t)t++];
+        out.push_back((int32_t)t);
+    }
+    out.push_back((int32_t)nt);
 }
 
 Batch::~Batch() {
@@ -226,7 +246,7 @@ Batch::~Batch() {
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     d_bd_.release(); d_tiles_.release(); d_list_.release();
     for (auto &c : d_conv_) c.release();
-    d_tmp_tiles_.release(); d_tmp_list_.release();
+    d_tmp_tiles_.release(); d_tmp_list_.release(); d_ranges_.release();
 }
 
 int Batch::active_count() const {
@@ -252,7 +272,7 @@ hm_status Batch::build_tables() {
     std::vector<int32_t> tiles, conv[3], list;
     pass_lds_ = 0;
     Fmax_ = 2;
-    any_wake_ = any_wake_e0_ = false;
+    any_wake_ = any_wake_e0_ = any_reduce_ = false;
     std_only_ = true;
     interp_lds_ = 0;
     for (size_t i = 0; i < rb_.size(); ++i) {
@@ -262,6 +282,7 @@ hm_status Batch::build_tables() {
         for (int t = 0; t < q.d.ntiles; ++t) tiles.push_back((int32_t)i);
         list.push_back((int32_t)i);
         if ((q.d.flags & kPhysMask) != kStdFlags) std_only_ = false;
+        if (q.d.flags & (F_WAKE | F_E0)) any_reduce_ = true;
         if (q.d.flags & F_WAKE) {
             any_wake_ = true;
             if (q.d.flags & F_E0) {
@@ -285,7 +306,28 @@ hm_status Batch::build_tables() {
     }
     ntiles_ = (int)tiles.size();
     nlist_ = (int)list.size();
+    // Workgroup ranges: contiguous runs of the tile table with equal numbers
+    // of pairs (tiles of different bunches differ in size), for the pass and
+    // for the interpolation sums (which only count wake + update_E0 bunches).
+    std::vector<int64_t> wpass((size_t)ntiles_), winterp((size_t)ntiles_);
+    for (int t = 0; t < ntiles_; ++t) {
+        const BunchDev &d = rb_[(size_t)tiles[(size_t)t]].d;
+        const int64_t lt = t - d.tile_begin;
+        const int64_t pairs = std::min(d.pairs_per_tile, d.npairs - lt * d.pairs_per_tile);
+        wpass[(size_t)t] = pairs;
+        winterp[(size_t)t] = ((d.flags & F_WAKE) && (d.flags & F_E0)) ? pairs : 0;
+    }
+    const int per_cu = interp_lds_ + 1024 <= (size_t)ctx_->lds_max / 2 ? 2 : 1;
+    pass_grid_ = std::min(ntiles_, ctx_->cus);
+    interp_grid_ = any_wake_e0_ ? std::min(ntiles_, per_cu * ctx_->cus) : 0;
+    std::vector<int32_t> ranges;
+    balanced_ranges(wpass, pass_grid_, ranges);
+    balanced_ranges(winterp, interp_grid_, ranges);
     hipStream_t s = ctx_->stream;
+    HM_TRY(d_ranges_.ensure(std::max<size_t>(ranges.size(), 1)));
+    if (!ranges.empty())
+        HIP_TRY(hipMemcpyAsync(d_ranges_.p, ranges.data(), ranges.size() * 4, hipMemcpyHostToDevice,
+                               s));
     HM_TRY(d_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
     HM_TRY(d_list_.ensure(std::max<size_t>(list.size(), 1)));
     for (int k = 0; k < 3; ++k) {
@@ -570,20 +612,19 @@ hm_status Batch::one_turn() {
         ev_used_ += 3;
         HIP_TRY(hipEventRecord(e0, s));
     }
-    HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, ntiles_, std::min(ntiles_, ctx_->cus), pass_lds_,
+    HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, d_ranges_.p, pass_grid_, pass_lds_,
                              std_only_ ? 1 : 0, s));
     if (timing_) HIP_TRY(hipEventRecord(e1, s));
+    if (any_reduce_) HIP_TRY(launch_bin_reduce(d_bd_.p, d_list_.p, nlist_, s));
     if (any_wake_) {
         HIP_TRY(launch_conv(CONV_FUSED, d_bd_.p, d_conv_[CONV_FUSED].p, nconv_[CONV_FUSED], Fmax_, s));
         HIP_TRY(launch_conv(CONV_FWD, d_bd_.p, d_conv_[CONV_FWD].p, nconv_[CONV_FWD], Fmax_, s));
         HIP_TRY(launch_conv(CONV_INV, d_bd_.p, d_conv_[CONV_INV].p, nconv_[CONV_INV], Fmax_, s));
     }
     if (timing_) HIP_TRY(hipEventRecord(e2, s));
-    if (any_wake_e0_) {   // two workgroups per CU when both windows fit the LDS
-        const int per_cu = interp_lds_ + 1024 <= (size_t)ctx_->lds_max / 2 ? 2 : 1;
-        HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, ntiles_,
-                                  std::min(ntiles_, per_cu * ctx_->cus), interp_lds_, s));
-    }
+    if (any_wake_e0_)   // two workgroups per CU when both windows fit the LDS
+        HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, d_ranges_.p + pass_grid_ + 1, interp_grid_,
+                                  interp_lds_, s));
     HIP_TRY(launch_finalize(d_bd_.p, d_list_.p, nlist_, s));
     return HM_OK;
 }

@@ -168,12 +168,17 @@ class Batch {
     std::vector<bool> used_;
     DevBuf<BunchDev> d_bd_;
     DevBuf<int32_t> d_tiles_, d_list_, d_tmp_tiles_, d_tmp_list_;
+    // per-workgroup tile ranges of the pass and the interpolation sums,
+    // [pass_grid_ + 1] then [interp_grid_ + 1] entries, balanced by pairs
+    DevBuf<int32_t> d_ranges_;
+    int pass_grid_ = 0, interp_grid_ = 0;
     DevBuf<int32_t> d_conv_[3];   // fused, forward, inverse convolution items
     int ntiles_ = 0, nlist_ = 0;
     int nconv_[3] = {0, 0, 0};
     size_t pass_lds_ = 0;
     int Fmax_ = 2;
     bool any_wake_ = false, any_wake_e0_ = false;   // some bunch with wake / wake + update_E0
+    bool any_reduce_ = false;                        // some bunch with wake or update_E0
     bool std_only_ = true;                          // all active bunches have kStdFlags
     size_t interp_lds_ = 0;                         // dynamic LDS of the interpolation sums
     bool dirty_ = true;

@@ -7,9 +7,11 @@
 //                 :58-71), SR damping (synchrotron_radiation.jl:44-56) and the
 //                 drift (evolution.jl:330-347); it bins the new z into an
 //                 LDS-privatised histogram (utils.jl:351-401) and writes
-//                 deterministic per-tile partial sums.
-//   k_conv_*      the bin phase: sigma_z, the wake current (evolution.jl:373-386)
-//                 and the line-density convolution with the wake
+//                 deterministic per-wave partial sums of each tile.
+//   k_bin_reduce  one workgroup per bunch reduces those sums: sigma_z, the
+//                 wake current (evolution.jl:373-386), the binned count and
+//                 the sum of the energy changes.
+//   k_conv_*      the line-density convolution with the wake
 //                 (wakefield.jl:186-319) by blocked LDS FFTs.
 //   k_interp_sum  (update_E0 with the wake) sum of the interpolated potential
 //                 over the post-drift z, which the E0 update of
@@ -369,6 +371,23 @@ __device__ __forceinline__ void st2(g_f64x2 *a, double x, double y) {
     v.y = y;
     *a = v;
 }
+// streaming (nontemporal) 16-byte load: for arrays far larger than the caches
+__device__ __forceinline__ double2 ldnt(const g_f64x2 *a) {
+    const f64x2 v = __builtin_nontemporal_load(a);
+    return make_double2(v.x, v.y);
+}
+// streaming (nontemporal) 16-byte store
+__device__ __forceinline__ void stnt(g_f64x2 *a, double x, double y) {
+    f64x2 v;
+    v.x = x;
+    v.y = y;
+    __builtin_nontemporal_store(v, a);
+}
+// P[k]*(1-w) + P[k+1]*w from the neighbour-pair table entry (P[k], P[k+1])
+__device__ __forceinline__ double gather_interp(const g_f64x2 *q, double w) {
+    const f64x2 v = *q;
+    return v.x * (1.0 - w) + v.y * w;
+}
 
 __device__ __forceinline__ uint8_t *pass_lds() {
     extern __shared__ __align__(16) uint8_t s_lds[];
@@ -391,22 +410,27 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
     }
 }
 
-// One tile of the fused pass.  FX = kStdFlags is the common configuration
+// The fused pass over a run of tiles.  FX = kStdFlags is the common configuration
 // with the flags folded at compile time; FX = 0 handles any other.  Both
 // evaluate the same expressions, so a bunch gets the same bits from either.
 // LH: histogram in LDS at offset 0; the window of P follows at byte hb.
+//
+// One call covers the run of tiles [lt0, lt1) of bunch B that this workgroup
+// owns.  Tiles are whole multiples of the block size, so thread t's pairs in
+// tile lt are lt*ppt + t + k*blockDim.x in ascending k, exactly as for a
+// tile taken alone: the loop keeps its one-pair prefetch running across the
+// tile boundaries and writes each tile's per-wave sums at the tile's end.
 template <uint32_t FX, bool LH>
-__device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, int hb) {
+__device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt1, int hb) {
     PassCtx c;
     load_ctx(B, c);
     const Flags<FX> fl{c.flags};
     const bool wake = fl.has(F_WAKE);
     const TurnScalars &sc = *B.sc;
     const int64_t n = B.n;
-    const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
-    const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
+    const int64_t ppt = B.pairs_per_tile;
     const int64_t full = n >> 1;                    // pairs with two particles
-    const int64_t pf = min(p1, full);
+    const int64_t pf_run = min((int64_t)lt1 * ppt, full);
     const uint32_t tabs = (uint32_t)(B.turn_offset + sc.turn);
     const bool inj = fl.has(F_INJ);
     const bool qe = fl.has(F_QE);
@@ -420,97 +444,108 @@ __device__ __forceinline__ void turn_pass_tile(const BunchDev &B, int lt, int hb
     g_f64 *zp = (g_f64 *)B.z;
     g_f64 *ep = (g_f64 *)B.dE;
     g_u32 *ghist = (g_u32 *)hist_of(B, sc.turn);
-    double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
 
-    int64_t p = p0 + threadIdx.x;
     double2 zn = make_double2(0.0, 0.0), en = make_double2(0.0, 0.0);
-    if (p < pf) {
-        zn = ld2((const g_f64x2 *)(zp + 2 * p));
-        en = ld2((const g_f64x2 *)(ep + 2 * p));
+    {
+        const int64_t p = (int64_t)lt0 * ppt + threadIdx.x;
+        if (p < pf_run) {
+            zn = ldnt((const g_f64x2 *)(zp + 2 * p));
+            en = ldnt((const g_f64x2 *)(ep + 2 * p));
+        }
     }
-    for (; p < pf; p += blockDim.x) {
-        const double2 zz = zn, ee = en;
-        const int64_t pn = p + blockDim.x;
-        if (pn < pf) {   // prefetch the next pair
-            zn = ld2((const g_f64x2 *)(zp + 2 * pn));
-            en = ld2((const g_f64x2 *)(ep + 2 * pn));
-        }
-        // the previous turn's potential at (P[k], P[k+1]): read from the LDS
-        // window (clamped index, always valid), then replaced by one 16-byte
-        // global gather for the few particles outside it
-        int k0 = 0, k1 = 0;
-        double w0 = 0.0, w1 = 0.0;
-        double2 q0 = make_double2(0.0, 0.0), q1 = make_double2(0.0, 0.0);
-        if (kick2) {
-            interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
-            interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
-            const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
-            if (wpairs) {
-                const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
-                q0 = make_double2(sP[i0], sP[i0 + 1]);
-                q1 = make_double2(sP[i1], sP[i1 + 1]);
+    for (int lt = lt0; lt < lt1; ++lt) {
+        const int64_t p0 = (int64_t)lt * ppt;
+        const int64_t p1 = min(p0 + ppt, B.npairs);
+        const int64_t pf = min(p1, full);
+        double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
+        for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) {
+            const double2 zz = zn, ee = en;
+            const int64_t pn = p + blockDim.x;
+            if (pn < pf_run) {   // prefetch the next pair (the next tile's first at a tile end)
+                zn = ldnt((const g_f64x2 *)(zp + 2 * pn));
+                en = ldnt((const g_f64x2 *)(ep + 2 * pn));
             }
-            if (j0 >= wpairs) q0 = ld2(P2 + k0);
-            if (j1 >= wpairs) q1 = ld2(P2 + k1);
-        }
-        double r0 = 0.0, r1 = 0.0;
-        if (qe) {
-            if (inj) {   // turn rows of an odd bunch are only 8-byte aligned
-                r0 = nrm[2 * p];
-                r1 = nrm[2 * p + 1];
-            } else {
-                qe_pair_normals(seed, (uint64_t)p, tabs, r0, r1);
-            }
-        }
-        double z0 = zz.x, z1 = zz.y, e0 = ee.x, e1 = ee.y;
-        if (c.pending_kick) {
+            // the previous turn's kick, interpolated between P[k] and P[k+1]:
+            // from the LDS window (clamped index, always valid), recomputed
+            // inside the branch from one 16-byte global gather for the few
+            // particles outside it.  The gather's wait stays in that branch, so
+            // the prefetch above remains in flight through the turn.
+            double kk0 = 0.0, kk1 = 0.0;
             if (kick2) {
-                e0 = e0 - (q0.x * (1.0 - w0) + q0.y * w0);
-                e1 = e1 - (q1.x * (1.0 - w1) + q1.y * w1);
-            } else {
-                e0 = e0 - c.P[0];
-                e1 = e1 - c.P[0];
+                int k0, k1;
+                double w0, w1;
+                interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
+                interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
+                const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
+                if (wpairs) {
+                    const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
+                    kk0 = sP[i0] * (1.0 - w0) + sP[i0 + 1] * w0;
+                    kk1 = sP[i1] * (1.0 - w1) + sP[i1 + 1] * w1;
+                }
+                if (j0 >= wpairs) kk0 = gather_interp(P2 + k0, w0);
+                if (j1 >= wpairs) kk1 = gather_interp(P2 + k1, w1);
             }
+            double r0 = 0.0, r1 = 0.0;
+            if (qe) {
+                if (inj) {   // turn rows of an odd bunch are only 8-byte aligned
+                    r0 = nrm[2 * p];
+                    r1 = nrm[2 * p + 1];
+                } else {
+                    qe_pair_normals(seed, (uint64_t)p, tabs, r0, r1);
+                }
+            }
+            double z0 = zz.x, z1 = zz.y, e0 = ee.x, e1 = ee.y;
+            if (c.pending_kick) {
+                if (kick2) {
+                    e0 = e0 - kk0;
+                    e1 = e1 - kk1;
+                } else {
+                    e0 = e0 - c.P[0];
+                    e1 = e1 - c.P[0];
+                }
+            }
+            if (c.pending_m) {
+                e0 = e0 - c.m_prev;
+                e1 = e1 - c.m_prev;
+            }
+            double d0, d1;
+            advance(c, fl, z0, e0, r0, d0);
+            advance(c, fl, z1, e1, r1, d1);
+            if (wake) {
+                bin_and_sum<LH>(c, ghist, z0, v);
+                bin_and_sum<LH>(c, ghist, z1, v);
+            }
+            v[2] += d0;
+            v[2] += d1;
+            // streaming stores: the bunch is far larger than the caches, which
+            // are left to the bin phase's tables
+            stnt((g_f64x2 *)(zp + 2 * p), z0, z1);
+            stnt((g_f64x2 *)(ep + 2 * p), e0, e1);
         }
-        if (c.pending_m) {
-            e0 = e0 - c.m_prev;
-            e1 = e1 - c.m_prev;
+        // the single particle of an odd bunch (last pair)
+        if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1) {
+            const int64_t i = n - 1;
+            double z0 = zp[i], e0 = ep[i], r0 = 0.0, r1 = 0.0;
+            if (qe) {
+                if (inj) r0 = nrm[i];
+                else qe_pair_normals(seed, (uint64_t)full, tabs, r0, r1);
+            }
+            e0 = apply_pending(c, z0, e0);
+            double d0;
+            advance(c, fl, z0, e0, r0, d0);
+            if (wake) bin_and_sum<LH>(c, ghist, z0, v);
+            v[2] += d0;
+            zp[i] = z0;
+            ep[i] = e0;
         }
-        double d0, d1;
-        advance(c, fl, z0, e0, r0, d0);
-        advance(c, fl, z1, e1, r1, d1);
-        if (wake) {
-            bin_and_sum<LH>(c, ghist, z0, v);
-            bin_and_sum<LH>(c, ghist, z1, v);
+        if (!wake) { v[0] = 0.0; v[1] = 0.0; }
+        // per-wave partial sums: no workgroup barrier per tile, so a wave never
+        // waits for the others of its workgroup between tiles
+        wave_sum<4>(v);
+        if ((threadIdx.x & 63) == 0) {
+            double *o = B.part + 4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6));
+            o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
         }
-        v[2] += d0;
-        v[2] += d1;
-        st2((g_f64x2 *)(zp + 2 * p), z0, z1);
-        st2((g_f64x2 *)(ep + 2 * p), e0, e1);
-    }
-    // the single particle of an odd bunch (last pair)
-    if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1) {
-        const int64_t i = n - 1;
-        double z0 = zp[i], e0 = ep[i], r0 = 0.0, r1 = 0.0;
-        if (qe) {
-            if (inj) r0 = nrm[i];
-            else qe_pair_normals(seed, (uint64_t)full, tabs, r0, r1);
-        }
-        e0 = apply_pending(c, z0, e0);
-        double d0;
-        advance(c, fl, z0, e0, r0, d0);
-        if (wake) bin_and_sum<LH>(c, ghist, z0, v);
-        v[2] += d0;
-        zp[i] = z0;
-        ep[i] = e0;
-    }
-    if (!wake) { v[0] = 0.0; v[1] = 0.0; }
-    // per-wave partial sums: no workgroup barrier per tile, so a wave never
-    // waits for the others of its workgroup between tiles
-    wave_sum<4>(v);
-    if ((threadIdx.x & 63) == 0) {
-        double *o = B.part + 4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6));
-        o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
     }
 }
 
@@ -542,22 +577,25 @@ __device__ inline void stage_bunch(const BunchDev &B) {
 }
 
 // The fused pass over a tile table.  Each workgroup takes a contiguous range
-// of tiles (one workgroup per CU), stages a bunch's LDS once for all of its
-// tiles in the range and flushes the histogram when it moves on.  The
-// per-tile sums depend on the tile and the block size only.
+// of tiles with about the same number of pairs as every other workgroup's
+// (one workgroup per CU), stages a bunch's LDS once for its run of
+// tiles in the range, streams the run and flushes the histogram when it moves
+// on.  The per-tile sums depend on the tile and the block size only.
 template <uint32_t FX>
 __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bds,
                                                  const int32_t *__restrict__ tile_bunch,
-                                                 int ntiles) {
+                                                 const int32_t *__restrict__ ranges) {
     // kStdFlags has the wake on and F_GHIST off: its histogram is in LDS
     constexpr bool kLdsOnly = (FX & F_WAKE) && !(FX & F_GHIST);
-    const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t0 = (int)blockIdx.x * per;
-    const int t1 = min(ntiles, t0 + per);
+    const int t0 = ranges[blockIdx.x];
+    const int t1 = ranges[blockIdx.x + 1];
     int staged = -1;
-    for (int tile = t0; tile < t1; ++tile) {
+    for (int tile = t0; tile < t1;) {
         const int bi = tile_bunch[tile];
         const BunchDev &B = bds[bi];
+        // the bunch's tiles are contiguous in the table: this workgroup's run
+        // of them ends at the bunch's last tile or the end of the range
+        const int te = min(t1, B.tile_begin + B.ntiles);
         if (bi != staged) {
             if (staged >= 0) flush_hist(bds[staged]);
             stage_bunch(B);
@@ -565,23 +603,25 @@ __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bd
         }
         const int hb = pass_hist_bytes(B);
         if (kLdsOnly || hb)
-            turn_pass_tile<FX, true>(B, tile - B.tile_begin, hb);
+            turn_pass_run<FX, true>(B, tile - B.tile_begin, te - B.tile_begin, hb);
         else
-            turn_pass_tile<FX, false>(B, tile - B.tile_begin, hb);
+            turn_pass_run<FX, false>(B, tile - B.tile_begin, te - B.tile_begin, hb);
+        tile = te;
     }
     if (staged >= 0) flush_hist(bds[staged]);
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__restrict__ bds,
                                                             const int32_t *__restrict__ tile_bunch,
-                                                            int ntiles) {
-    turn_pass_blocks<0>(bds, tile_bunch, ntiles);
+                                                            const int32_t *__restrict__ ranges) {
+    turn_pass_blocks<0>(bds, tile_bunch, ranges);
 }
 
 // the same for tables whose bunches all have exactly kStdFlags
 __global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
-    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch, int ntiles) {
-    turn_pass_blocks<kStdFlags>(bds, tile_bunch, ntiles);
+    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
+    const int32_t *__restrict__ ranges) {
+    turn_pass_blocks<kStdFlags>(bds, tile_bunch, ranges);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,39 +635,87 @@ __global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
 // since each block product g_a * W_j is real, has length 2B-1 <= F (no
 // wrap), and lands on output blocks a+j (first half) and a+j+1 (second half).
 
-struct BinScalars {
-    double sz, cur, nh;
-    int centre;   // interpolation cell of the centroid
-    double mz;    // centroid
-};
+// Sums over the n entries of a per-wave partial array, `NV` doubles per
+// entry (NV = 1 or 4).  Each thread adds entries threadIdx.x + k*blockDim.x
+// in ascending k, eight loads in flight; the workgroup sum follows in
+// block_sum's fixed order, so the result depends on n and the block size
+// only.
+template <int NV>
+__device__ inline void sum_partials(const double *a, int n, double (&v)[NV]) {
+    const int st = (int)blockDim.x;
+    int t = (int)threadIdx.x;
+    if constexpr (NV == 4) {
+        const g_f64x2 *q = (const g_f64x2 *)a;
+        for (; t + 7 * st < n; t += 8 * st) {
+            double2 x[8], y[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                x[u] = ld2(q + 2 * (t + u * st));
+                y[u] = ld2(q + 2 * (t + u * st) + 1);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                v[0] += x[u].x;
+                v[1] += x[u].y;
+                v[2] += y[u].x;
+                v[3] += y[u].y;
+            }
+        }
+        for (; t < n; t += st) {
+            const double2 x = ld2(q + 2 * t), y = ld2(q + 2 * t + 1);
+            v[0] += x.x;
+            v[1] += x.y;
+            v[2] += y.x;
+            v[3] += y.y;
+        }
+    } else {
+        const g_f64 *q = (const g_f64 *)a;
+        for (; t + 7 * st < n; t += 8 * st) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = q[t + u * st];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[0] += x[u];
+        }
+        for (; t < n; t += st) v[0] += q[t];
+    }
+}
 
-// sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386), the
-// binned count, the centroid and its cell from the tile sums; valid in every
-// thread.  s_bc holds 5 doubles.
-__device__ inline BinScalars bin_scalars(const BunchDev &B, double *s_red, double *s_bc) {
-    const TurnScalars &sc = *B.sc;
-    double v[3] = {0.0, 0.0, 0.0};
-    for (int t = threadIdx.x; t < B.ntiles * kTileWaves; t += blockDim.x) {
-        v[0] += B.part[4 * t];
-        v[1] += B.part[4 * t + 1];
-        v[2] += B.part[4 * t + 3];
+// Once per turn and bunch, after the pass: the pass's per-wave sums reduced
+// into the turn scalars.  sum (dE'-dE_init) for the E0 update; for a wake
+// bunch sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386),
+// the binned count, and the interpolation cell of the centroid (where the LDS
+// windows of P are placed; the placement changes no result); the turn
+// monitor's centroid, sigma_z and binned count.
+__global__ __launch_bounds__(kReduceThreads) void k_bin_reduce(const BunchDev *__restrict__ bds,
+                                                               const int32_t *__restrict__ list) {
+    __shared__ double s_red[4 * 16];
+    const BunchDev &B = bds[list[blockIdx.x]];
+    if (!(B.flags & (F_WAKE | F_E0))) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};   // sum z, sum z^2, sum (dE'-dE_init), binned count
+    sum_partials<4>(B.part, B.ntiles * kTileWaves, v);
+    block_sum<4>(v, s_red);
+    if (threadIdx.x != 0) return;
+    TurnScalars &sc = *B.sc;
+    sc.sum_dE = v[2];
+    if (!(B.flags & F_WAKE)) return;
+    const double sz = gstd_from_sums(v[0], v[1], (double)B.n);
+    const double mz = v[0] / (double)B.n;
+    sc.sigma_z = sz;
+    sc.current = (B.flags & F_CHARGE)
+                     ? B.charge
+                     : (((B.pf / sc.E0) / (kTwoPi * B.radius)) * sz) /
+                           (sc.eta0 * (B.sigma_E0 * B.sigma_E0));
+    sc.n_h = v[3];
+    const double kc = floor((mz - B.c_first) * B.inv_s);
+    const double top = (double)(B.nbins - 1);
+    sc.centre_bin = (kc == kc) ? (int)(kc < 0.0 ? 0.0 : (kc > top ? top : kc)) : 0;
+    if (B.mon) {   // turn monitor: centroid and sigma_z after the drift, binned count
+        double *m = B.mon + 4 * sc.turn;
+        m[0] = mz;
+        m[1] = sz;
+        m[3] = v[3];
     }
-    block_sum<3>(v, s_red);
-    if (threadIdx.x == 0) {
-        const double sz = gstd_from_sums(v[0], v[1], (double)B.n);
-        s_bc[0] = sz;
-        s_bc[1] = (B.flags & F_CHARGE)
-                      ? B.charge
-                      : (((B.pf / sc.E0) / (kTwoPi * B.radius)) * sz) /
-                            (sc.eta0 * (B.sigma_E0 * B.sigma_E0));
-        s_bc[2] = v[2];
-        const double kc = floor((v[0] / (double)B.n - B.c_first) * B.inv_s);
-        const double top = (double)(B.nbins - 1);
-        s_bc[3] = (kc == kc) ? (kc < 0.0 ? 0.0 : (kc > top ? top : kc)) : 0.0;
-        s_bc[4] = v[0] / (double)B.n;
-    }
-    __syncthreads();
-    return BinScalars{s_bc[0], s_bc[1], s_bc[2], (int)s_bc[3], s_bc[4]};
 }
 
 __device__ __forceinline__ double line_density(const BunchDev &B, const uint32_t *h, int i,
@@ -692,61 +780,33 @@ __device__ inline void zero_block(const BunchDev &B, int o) {
 __global__ __launch_bounds__(kConvThreads) void k_conv_fused(const BunchDev *__restrict__ bds,
                                                              const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
-    __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[5];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int o = items[2 * blockIdx.x + 1];
-    const BinScalars bs = bin_scalars(B, s_red, s_bc);
-    if (o == 0 && threadIdx.x == 0) {
-        B.sc->sigma_z = bs.sz;
-        B.sc->current = bs.cur;
-        B.sc->n_h = bs.nh;
-        B.sc->centre_bin = bs.centre;
-        if (B.mon) {   // turn monitor: centroid and sigma_z after the drift, binned count
-            double *m = B.mon + 4 * B.sc->turn;
-            m[0] = bs.mz;
-            m[1] = bs.sz;
-            m[3] = bs.nh;
-        }
-    }
+    const double nh = B.sc->n_h, cur = B.sc->current;
     clear_next_hist(B, o);
-    if (bs.nh == 0.0 || B.nJ == 0) {   // wakefield.jl:305-311
+    if (nh == 0.0 || B.nJ == 0) {   // wakefield.jl:305-311
         zero_block(B, o);
         return;
     }
     const double2 *tw = reinterpret_cast<const double2 *>(B.tw);
     const double2 *What = reinterpret_cast<const double2 *>(B.What);
-    load_packed(sX, B, o, 1.0 / bs.nh);
+    load_packed(sX, B, o, 1.0 / nh);
     fft_dit_forward(sX, B.F, B.logF, tw);
     for (int k = threadIdx.x; k < B.F; k += blockDim.x) sX[k] = cmul(sX[k], What[k]);
     fft_dif_inverse(sX, B.F, B.logF, tw);
-    store_block(B, sX, o, bs.cur);
+    store_block(B, sX, o, cur);
 }
 
 // phase A of the blocked convolution: X_a = FFT(g_a + i g_{a-1})
 __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__restrict__ bds,
                                                            const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
-    __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[5];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int a = items[2 * blockIdx.x + 1];
-    const BinScalars bs = bin_scalars(B, s_red, s_bc);
-    if (a == 0 && threadIdx.x == 0) {
-        B.sc->sigma_z = bs.sz;
-        B.sc->current = bs.cur;
-        B.sc->n_h = bs.nh;
-        B.sc->centre_bin = bs.centre;
-        if (B.mon) {   // turn monitor: centroid and sigma_z after the drift, binned count
-            double *m = B.mon + 4 * B.sc->turn;
-            m[0] = bs.mz;
-            m[1] = bs.sz;
-            m[3] = bs.nh;
-        }
-    }
+    const double nh = B.sc->n_h;
     clear_next_hist(B, a);
-    if (bs.nh == 0.0) return;
-    load_packed(sX, B, a, 1.0 / bs.nh);
+    if (nh == 0.0) return;
+    load_packed(sX, B, a, 1.0 / nh);
     fft_dit_forward(sX, B.F, B.logF, reinterpret_cast<const double2 *>(B.tw));
     double2 *X = reinterpret_cast<double2 *>(B.X) + (int64_t)a * B.F;
     for (int k = threadIdx.x; k < B.F; k += blockDim.x) X[k] = sX[k];
@@ -757,12 +817,10 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
 __global__ __launch_bounds__(kConvInvThreads) void k_conv_inv(const BunchDev *__restrict__ bds,
                                                               const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
-    __shared__ double s_red[3 * 16];
-    __shared__ double s_bc[5];
     const BunchDev &B = bds[items[2 * blockIdx.x]];
     const int o = items[2 * blockIdx.x + 1];
-    const BinScalars bs = bin_scalars(B, s_red, s_bc);
-    if (bs.nh == 0.0) {
+    const double nh = B.sc->n_h, cur = B.sc->current;
+    if (nh == 0.0) {
         zero_block(B, o);
         return;
     }
@@ -793,7 +851,7 @@ __global__ __launch_bounds__(kConvInvThreads) void k_conv_inv(const BunchDev *__
         sX[k] = acc;
     }
     fft_dif_inverse(sX, F, B.logF, reinterpret_cast<const double2 *>(B.tw));
-    store_block(B, sX, o, bs.cur);
+    store_block(B, sX, o, cur);
 }
 
 // FFT of one wake block per workgroup (set-up of an evolve call)
@@ -816,79 +874,131 @@ __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__res
 // E0 update support, moments, init, finalize
 // ---------------------------------------------------------------------------
 
-// Sum over a tile of the interpolated potential at the post-drift z (the
-// E0 update of a wake bunch).  Each workgroup walks a contiguous range of
-// tiles and stages, once per bunch it meets, a window of B.iwin entries of P
-// around the centroid in LDS (the whole table unless it exceeds the LDS); the
-// particles outside it read P from global memory.  Each thread adds its pairs in ascending order, so
-// the per-tile sums depend on the tile and the block size only.
-__global__ __launch_bounds__(kInterpThreads) void k_interp_sum(
-    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch, int ntiles) {
-    extern __shared__ double sP[];   // B.iwin entries of P
-    const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t0 = (int)blockIdx.x * per;
-    const int t1 = min(ntiles, t0 + per);
-    int staged = -1, wlo = 0;
-    unsigned wpairs = 0;
-    for (int tile = t0; tile < t1; ++tile) {
-        const int bi = tile_bunch[tile];
-        const BunchDev &B = bds[bi];
-        if (!((B.flags & F_WAKE) && (B.flags & F_E0))) continue;
-        const int nb = B.nbins;
-        const double *__restrict__ P = B.P;
-        if (bi != staged) {
-            const int w = B.iwin;
-            wlo = window_lo(B.sc->centre_bin, w, nb);
-            wpairs = (unsigned)(w - 1);
-            __syncthreads();
-            for (int k = threadIdx.x; k < w; k += blockDim.x) sP[k] = P[wlo + k];
-            __syncthreads();
-            staged = bi;
+// The z loads of the interpolation sums and their tile order within a run.
+// Streaming loads leave the last-level cache to other data; the backward order
+// would start each run on the tiles the pass wrote last.
+constexpr bool kInterpStreaming = true;
+constexpr bool kInterpBackward = false;
+__device__ __forceinline__ double2 ldz(const g_f64x2 *a) {
+    return kInterpStreaming ? ldnt(a) : ld2(a);
+}
+
+// Sums over a run of tiles [lt0, lt1) of bunch B of the interpolated
+// potential at the post-drift z (the E0 update of a wake bunch), one sum per
+// tile and wave.  The run first stages B.iwin entries of P around the
+// centroid in LDS (the whole table unless it exceeds the LDS); particles
+// outside the window read the pair table P2.  Each thread takes its pairs
+// of a tile in groups of four, in ascending order, with the next group's
+// streaming loads (the next tile's first group at a tile's end) issued before
+// this group's sums; the group loop has the same trip count in every lane.
+// A tile's sums therefore depend on the tile and the block size only.
+__device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
+    extern __shared__ double sP[];
+    const int nb = B.nbins;
+    const double *__restrict__ P = B.P;
+    const int w = B.iwin;
+    const int wlo = window_lo(B.sc->centre_bin, w, nb);
+    const unsigned wpairs = (unsigned)(w - 1);
+    __syncthreads();   // the previous run's window is no longer read
+    for (int k = threadIdx.x; k < w; k += blockDim.x) sP[k] = P[wlo + k];
+    __syncthreads();
+    const int64_t n = B.n, ppt = B.pairs_per_tile, full = n >> 1;
+    const int64_t bs = blockDim.x;
+    const int K = (int)(ppt / bs);   // pairs of each thread in a full tile
+    const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
+    const g_f64 *zp = (const g_f64 *)B.z;
+    const g_f64x2 *P2 = (const g_f64x2 *)B.P2;
+    const g_f64x2 *zpair = (const g_f64x2 *)zp;
+    const int tid = (int)threadIdx.x, ibs = (int)bs;
+    // pair offsets within a tile are 32-bit; tile bases are uniform
+    // The ring's loads are issued unconditionally (a lane without a next pair
+    // reads pair 0 and ignores it), so that every wait for a slot is the same
+    // count of younger loads.  Needs a two-particle pair: full >= 1.
+    const bool ring = nb >= 2 && full >= 1;
+    const int nrun = lt1 - lt0;
+    // the j-th tile of the run, and its two-particle pairs
+    auto tile_of = [&](int j) { return kInterpBackward ? lt1 - 1 - j : lt0 + j; };
+    auto lim_of = [&](int lt) {
+        const int64_t p0 = (int64_t)lt * ppt;
+        return (int)(min(min(p0 + ppt, B.npairs), full) - p0);
+    };
+    double2 zn[4];
+    if (ring) {
+        const int lt = tile_of(0), lim0 = lim_of(lt);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = tid + u * ibs;
+            zn[u] = ldz((u < K && r < lim0) ? zpair + (int64_t)lt * ppt + r : zpair);
         }
-        const int lt = tile - B.tile_begin;
-        const int64_t n = B.n;
-        const int64_t p0 = (int64_t)lt * B.pairs_per_tile;
-        const int64_t p1 = min(p0 + B.pairs_per_tile, B.npairs);
-        const int64_t full = n >> 1;
-        const int64_t pf = min(p1, full);
-        const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
-        const g_f64 *zp = (const g_f64 *)B.z;
-        const g_f64 *gP = (const g_f64 *)P;
+    }
+    for (int j = 0; j < nrun; ++j) {
+        const int lt = tile_of(j);
+        const int64_t p0 = (int64_t)lt * ppt;
+        const int lim = lim_of(lt);   // two-particle pairs of this tile
         double v[1] = {0.0};
-        if (nb >= 2) {
-            // four pairs per thread in flight; P from the LDS window (clamped
-            // index), replaced by global reads outside it
-            const int64_t bs = blockDim.x;
-            for (int64_t p = p0 + threadIdx.x; p < pf; p += 4 * bs) {
-                double2 zz[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    zz[u] = make_double2(0.0, 0.0);
-                    if (p + u * bs < pf) zz[u] = ld2((const g_f64x2 *)(zp + 2 * (p + u * bs)));
+        if (ring) {
+            for (int k = 0; k < K; k += 4) {
+                // the next group: the rest of this tile, else the next tile's first
+                const g_f64x2 *nbase = zpair + p0;
+                int nr = 0, nlim = 0, nk = 0;
+                if (k + 4 < K) {
+                    nr = tid + (k + 4) * ibs;
+                    nlim = lim;
+                    nk = K - (k + 4);
+                } else if (j + 1 < nrun) {
+                    const int ltn = tile_of(j + 1);
+                    nbase = zpair + (int64_t)ltn * ppt;
+                    nr = tid;
+                    nlim = lim_of(ltn);
+                    nk = K;
                 }
+                // a ring of four slots: slot u's next load is issued as soon as
+                // its current pair is taken
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (p + u * bs >= pf) break;
+                    const double2 zz = zn[u];
+                    const int r = nr + u * ibs;
+                    zn[u] = ldz((u < nk && r < nlim) ? nbase + r : zpair);
+                    if (k + u >= K || tid + (k + u) * ibs >= lim) continue;
                     int k0, k1;
                     double w0, w1;
-                    interp_index(zz[u].x, c0, c1, inv, nb, k0, w0);
-                    interp_index(zz[u].y, c0, c1, inv, nb, k1, w1);
+                    interp_index(zz.x, c0, c1, inv, nb, k0, w0);
+                    interp_index(zz.y, c0, c1, inv, nb, k1, w1);
                     const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
                     const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
-                    double a0 = sP[i0], b0 = sP[i0 + 1], a1 = sP[i1], b1 = sP[i1 + 1];
-                    if (j0 >= wpairs) { a0 = gP[k0]; b0 = gP[k0 + 1]; }
-                    if (j1 >= wpairs) { a1 = gP[k1]; b1 = gP[k1 + 1]; }
-                    v[0] += a0 * (1.0 - w0) + b0 * w0;
-                    v[0] += a1 * (1.0 - w1) + b1 * w1;
+                    double t0 = sP[i0] * (1.0 - w0) + sP[i0 + 1] * w0;
+                    double t1 = sP[i1] * (1.0 - w1) + sP[i1 + 1] * w1;
+                    // outside the window: the pair table, with the wait in the branch
+                    if (j0 >= wpairs) t0 = gather_interp(P2 + k0, w0);
+                    if (j1 >= wpairs) t1 = gather_interp(P2 + k1, w1);
+                    v[0] += t0;
+                    v[0] += t1;
                 }
             }
-        } else {
-            for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) v[0] += P[0] + P[0];
+        } else if (nb < 2) {
+            for (int64_t p = p0 + threadIdx.x; p < p0 + lim; p += bs) v[0] += P[0] + P[0];
         }
-        if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1)
+        if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < min(p0 + ppt, B.npairs))
             v[0] += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
         wave_sum<1>(v);
         if ((threadIdx.x & 63) == 0) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = v[0];
+    }
+}
+
+// The interpolation sums of the wake + update_E0 bunches over a tile table:
+// each workgroup walks its contiguous range of tiles, one run per bunch.  At
+// most 64 VGPRs, so that two workgroups fit a CU when both windows fit its LDS.
+__global__ __launch_bounds__(kInterpThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_interp_sum(
+    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
+    const int32_t *__restrict__ ranges) {
+    const int t0 = ranges[blockIdx.x];
+    const int t1 = ranges[blockIdx.x + 1];
+    for (int tile = t0; tile < t1;) {
+        const BunchDev &B = bds[tile_bunch[tile]];
+        const int te = min(t1, B.tile_begin + B.ntiles);
+        if ((B.flags & F_WAKE) && (B.flags & F_E0))
+            interp_sum_run(B, tile - B.tile_begin, te - B.tile_begin);
+        tile = te;
     }
 }
 
@@ -974,25 +1084,20 @@ __global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restri
     }
 }
 
-__global__ __launch_bounds__(kFinalThreads) void k_finalize(const BunchDev *__restrict__ bds,
-                                                            const int32_t *__restrict__ list) {
-    __shared__ double s_red[2 * 16];
+__global__ __launch_bounds__(kReduceThreads) void k_finalize(const BunchDev *__restrict__ bds,
+                                                             const int32_t *__restrict__ list) {
+    __shared__ double s_red[16];
     const BunchDev &B = bds[list[blockIdx.x]];
     const uint32_t fl = B.flags;
-    double v[2] = {0.0, 0.0};   // sum (dE'-dE_init), sum interpolated P
-    if (fl & F_E0) {
-        for (int t = threadIdx.x; t < B.ntiles * kTileWaves; t += blockDim.x) {
-            v[0] += B.part[4 * t + 2];
-            if (fl & F_WAKE) v[1] += B.ipart[t];
-        }
-    }
-    block_sum<2>(v, s_red);
+    double v[1] = {0.0};   // sum of the interpolated P (the sum of dE'-dE_init is in sc)
+    if ((fl & F_E0) && (fl & F_WAKE)) sum_partials<1>(B.ipart, B.ntiles * kTileWaves, v);
+    block_sum<1>(v, s_red);
     if (threadIdx.x == 0) {
         TurnScalars s = *B.sc;
         double E0 = s.E0;
         if (s.drift_skip) s.drift_skipped_count += 1;
         if (fl & F_E0) {                                  // evolution.jl:419-429
-            const double mean = (v[0] - v[1]) / (double)B.n;
+            const double mean = (s.sum_dE - v[0]) / (double)B.n;
             E0 = E0 + mean;
             s.m_prev = mean;
             s.pending_m = 1;
@@ -1253,11 +1358,21 @@ __global__ __launch_bounds__(256) void k_pair_sums_final(int mode, const double 
 // Plain device copy (16 B per lane per step, streaming loads and stores,
 // 64 workgroups of 1024 per CU): the HBM ceiling of a kernel that reads and
 // writes each byte once, and the calibration of the HBM byte counters.
+// grid-stride copy with four 16-byte loads of each thread in flight before
+// its stores (the fastest form of a sweep over grid, block size and loads in
+// flight on MI355X)
 __global__ __launch_bounds__(1024) void k_stream_copy(const f64x2 *__restrict__ src,
                                                       f64x2 *__restrict__ dst, int64_t n2) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
-         i += (int64_t)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += 4 * st) {
+        f64x2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * st < n2) v[u] = __builtin_nontemporal_load(src + i + u * st);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i + u * st < n2) __builtin_nontemporal_store(v[u], dst + i + u * st);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1307,15 +1422,15 @@ hipError_t launch_init(const BunchDev *bds, const int32_t *list, int nb, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, int ntiles, int grid,
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, const int32_t *ranges, int grid,
                             size_t lds, int std_only, hipStream_t s) {
-    if (ntiles <= 0 || grid <= 0) return hipSuccess;
+    if (grid <= 0) return hipSuccess;
     if (std_only)
         hipLaunchKernelGGL(k_turn_pass_std, dim3((unsigned)grid), dim3(kPassThreads), lds, s, bds,
-                           tb, ntiles);
+                           tb, ranges);
     else
         hipLaunchKernelGGL(k_turn_pass, dim3((unsigned)grid), dim3(kPassThreads), lds, s, bds, tb,
-                           ntiles);
+                           ranges);
     return hipGetLastError();
 }
 
@@ -1332,17 +1447,23 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
     return hipGetLastError();
 }
 
-hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, int ntiles, int grid,
-                             size_t lds_bytes, hipStream_t s) {
-    if (ntiles <= 0 || grid <= 0) return hipSuccess;
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, const int32_t *ranges,
+                             int grid, size_t lds_bytes, hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_interp_sum, dim3((unsigned)grid), dim3(kInterpThreads), lds_bytes, s, bds,
-                       tb, ntiles);
+                       tb, ranges);
     return hipGetLastError();
 }
 
 hipError_t launch_finalize(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
     if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(kFinalThreads), 0, s, bds, list);
+    hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(kReduceThreads), 0, s, bds, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kReduceThreads), 0, s, bds, list);
     return hipGetLastError();
 }
 

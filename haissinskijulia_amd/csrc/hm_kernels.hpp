@@ -12,7 +12,8 @@ namespace hm {
 constexpr int kPassThreads = 1024;     // turn-pass / moments / interp-sum block
 constexpr int kConvThreads = 256;      // convolution block
 constexpr int kConvInvThreads = 1024;  // phase-B convolution block (spectral sums in flight)
-constexpr int kFinalThreads = 256;     // per-bunch finalize block
+constexpr int kFinalThreads = 256;     // per-bunch set-up and moment blocks
+constexpr int kReduceThreads = 1024;   // per-bunch reductions of the per-wave tile sums
 constexpr int kInterpThreads = 1024;   // LDS-staged interpolation sums
 // partial sums per tile are kept per wave of the 1024-thread pass and
 // interpolation-sum workgroups: part[ntiles][kTileWaves][4], ipart[ntiles][kTileWaves]
@@ -35,18 +36,21 @@ hipError_t launch_moments(const BunchDev *bds, const int32_t *tile_bunch, int nt
 hipError_t launch_moments_final(const BunchDev *bds, const int32_t *bunch_list, int nb, int mode,
                                 hipStream_t s);
 hipError_t launch_init(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
-// `grid` workgroups over the tile table; lds_bytes = the largest per-bunch
-// histogram + potential window; std_only: every tile's bunch has
-// (flags & kPhysMask) == kStdFlags
-hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int grid,
-                            size_t lds_bytes, int std_only, hipStream_t s);
+// `grid` workgroups over the tile table, workgroup g taking tiles
+// [ranges[g], ranges[g+1]); lds_bytes = the largest per-bunch histogram +
+// potential window; std_only: every tile's bunch has (flags & kPhysMask) == kStdFlags
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *ranges,
+                            int grid, size_t lds_bytes, int std_only, hipStream_t s);
 // items are (bunch, block) pairs; Fmax is the largest FFT length among them
 hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
                        hipStream_t s);
 // interpolation sums of the wake + update_E0 bunches: `grid` workgroups over
-// the tile table; lds_bytes = 8 x the largest iwin among them
-hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int grid,
-                             size_t lds_bytes, hipStream_t s);
+// the tile table (ranges as for the pass); lds_bytes = 8 x the largest iwin
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *ranges,
+                             int grid, size_t lds_bytes, hipStream_t s);
+// per-bunch reduction of the pass's tile sums into the turn scalars (sum of
+// dE'-dE_init; sigma_z, current, binned count, centroid cell of a wake bunch)
+hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
 hipError_t launch_finalize(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
 // FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
 hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,

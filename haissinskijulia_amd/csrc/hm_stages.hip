@@ -167,16 +167,11 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     HM_TRY(ctx->sbuf[0].ensure((size_t)nbins));   // W
     HM_TRY(ctx->sbuf[1].ensure((size_t)nbins));   // lam
     HM_TRY(ctx->sbuf[2].ensure((size_t)nbins));   // P
-    // What, X, and the tile partials of one tile (kTileWaves x 4: only the
-    // binned count n_h is non-zero, the bin phase reads them all)
-    HM_TRY(ctx->sbuf[3].ensure(what + xs + 4 * kTileWaves));
+    HM_TRY(ctx->sbuf[3].ensure(what + xs));   // What, X
     HM_TRY(ctx->ssc.ensure(1));
     HM_TRY(ctx->sbd.ensure(1));
     double *What = ctx->sbuf[3].p;
-    double *part = What + what + xs;
     d.X = What + what;
-    double hp[4 * kTileWaves] = {};
-    hp[3] = (double)nh;
     const double *tw;
     HM_TRY(twiddles(ctx, d.F, &tw));
     d.hist = ctx->shist.p;   // counts in the first half (turn 0); the kernels clear the second
@@ -184,12 +179,13 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     d.P = ctx->sbuf[2].p;
     d.What = What;
     d.tw = tw;
-    d.part = part;
     d.sc = ctx->ssc.p;
-    TurnScalars sc;
+    TurnScalars sc;   // the bin phase reads the binned count and the current
     std::memset(&sc, 0, sizeof(sc));
     sc.E0 = 1.0;
     sc.eta0 = 1.0;
+    sc.n_h = (double)nh;
+    sc.current = current;
     std::vector<int32_t> items;
     for (int o = 0; o < d.nA; ++o) {
         items.push_back(0);
@@ -200,7 +196,6 @@ hm_status hm_wake_potential(hm_ctx *ctx, const int64_t *counts, int64_t nbins,
     HIP_TRY(hipMemcpyAsync(ctx->shist.p, h.data(), (size_t)nbins * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->sbuf[0].p, wake_tab, (size_t)nbins * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->sbuf[1].p, lam_tab, (size_t)nbins * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(part, hp, sizeof(hp), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->ssc.p, &sc, sizeof(sc), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->sbd.p, &d, sizeof(d), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(di.p, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));

@@ -52,7 +52,8 @@ struct TurnScalars {
     double damp;       // 1 - dU/dE (synchrotron_radiation.jl:44-45)
     double exc;        // sqrt(1-(1-dU/dE)^2) * sigma_E0 (quantum_excitation.jl:58-59)
     double m_prev;     // mean energy change still to subtract (evolution.jl:429-434)
-    double sigma_z, current, n_h;   // last bin phase (diagnostics)
+    double sigma_z, current, n_h;   // bin phase of this turn (wake bunches)
+    double sum_dE;     // sum over the bunch of dE'-dE_init of this turn (evolution.jl:419-429)
     int32_t pending_kick;   // 1: the previous turn's wake kick is still to be applied
     int32_t pending_m;      // 1: m_prev still to be subtracted
     int32_t drift_skip;     // apply_phase_advance! skip condition of this turn
