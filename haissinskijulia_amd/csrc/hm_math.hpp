@@ -1,9 +1,13 @@
-// hm_math.hpp -- FP64 elementary functions of the per-particle pass, written
-// from their series: sin/cos by Cody-Waite reduction to |r| <= pi/4 and
-// Taylor polynomials (truncation error below 1e-19), log by the atanh series
-// of s = (m-1)/(m+1) on the mantissa m in [sqrt(1/2), sqrt(2)).  Results are
-// within a few ulp of the correctly rounded values (tests/test_math.py runs
-// this same code compiled for the CPU against numpy).
+// hm_math.hpp -- FP64 elementary functions of the per-particle pass: sin/cos
+// by Cody-Waite reduction to |r| <= pi/4, log through atanh of
+// s = (m-1)/(m+1) on the mantissa m in [sqrt(1/2), sqrt(2)).  The polynomial
+// parts are interpolants at Chebyshev nodes (near-minimax; the method of
+// Chebyshev economization, e.g. Muller, "Elementary Functions", ch. 3), fitted
+// in 60-digit arithmetic to the remainder after the leading Taylor terms;
+// their truncation error is below 1e-19 on the reduced interval, so each needs
+// fewer terms than the Taylor series.  Results are within a few ulp of the
+// correctly rounded values (tests/test_math.py runs this same code compiled for
+// the CPU against numpy).
 //
 // The includer defines HM_QUAL (the function qualifier) and an Ops policy
 // with rcp(a) (an estimate of 1/a), frexp_mant(u) in [0.5, 1) and
@@ -34,31 +38,30 @@ constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
 constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
 constexpr double kSqrtHalf = 0.70710678118654752440;
 
-// sin r, |r| <= pi/4: r - r^3 (1/3! - r^2/5! + ... - r^14/17!)
+// sin r, |r| <= pi/4: r - r^3 p(r^2), p(t) ~ (r - sin r)/r^3 = 1/3! - t/5! + ...
+// (degree 6 in t, truncation error 6e-21 at pi/4)
 HM_QUAL double sin_poly(double r) {
     const double r2 = r * r;
-    double p = -1.0 / 355687428096000.0;               // -1/17!
-    p = HM_FMAK(p, r2,1.0 / 1307674368000.0);   // +1/15!
-    p = HM_FMAK(p, r2,-1.0 / 6227020800.0);     // -1/13!
-    p = HM_FMAK(p, r2,1.0 / 39916800.0);        // +1/11!
-    p = HM_FMAK(p, r2,-1.0 / 362880.0);         // -1/9!
-    p = HM_FMAK(p, r2,1.0 / 5040.0);            // +1/7!
-    p = HM_FMAK(p, r2,-1.0 / 120.0);            // -1/5!
-    p = HM_FMAK(p, r2,1.0 / 6.0);               // +1/3!
+    double p = 7.586697117706917984663e-13;
+    p = HM_FMAK(p, r2, -1.605853161898614683644e-10);
+    p = HM_FMAK(p, r2, 2.505210623244757732801e-8);
+    p = HM_FMAK(p, r2, -2.755731921933916651755e-6);
+    p = HM_FMAK(p, r2, 1.984126984126506494653e-4);
+    p = HM_FMAK(p, r2, -8.333333333333331492084e-3);
+    p = HM_FMAK(p, r2, 1.666666666666666666551e-1);
     return __builtin_fma(-(p * r2), r, r);
 }
 
-// cos r, |r| <= pi/4: 1 - r^2/2 + r^4 (1/4! - r^2/6! + ... - r^14/18!)
+// cos r, |r| <= pi/4: 1 - r^2/2 + r^4 p(r^2), p(t) ~ (cos r - 1 + t/2)/t^2 =
+// 1/4! - t/6! + ... (degree 5 in t, truncation error 5e-19 at pi/4)
 HM_QUAL double cos_poly(double r) {
     const double r2 = r * r;
-    double p = -1.0 / 6402373705728000.0;              // -1/18!
-    p = HM_FMAK(p, r2,1.0 / 20922789888000.0);  // +1/16!
-    p = HM_FMAK(p, r2,-1.0 / 87178291200.0);    // -1/14!
-    p = HM_FMAK(p, r2,1.0 / 479001600.0);       // +1/12!
-    p = HM_FMAK(p, r2,-1.0 / 3628800.0);        // -1/10!
-    p = HM_FMAK(p, r2,1.0 / 40320.0);           // +1/8!
-    p = HM_FMAK(p, r2,-1.0 / 720.0);            // -1/6!
-    p = HM_FMAK(p, r2,1.0 / 24.0);              // +1/4!
+    double p = -1.138263242552171800541e-11;
+    p = HM_FMAK(p, r2, 2.087614626840319789335e-9);
+    p = HM_FMAK(p, r2, -2.755731727172979288199e-7);
+    p = HM_FMAK(p, r2, 2.480158729876568792727e-5);
+    p = HM_FMAK(p, r2, -1.388888888888739723669e-3);
+    p = HM_FMAK(p, r2, 4.166666666666666538875e-2);
     // 1 - r^2/2 evaluated as w + (exact remainder) to keep the last bits
     const double hr2 = 0.5 * r2;
     const double w = 1.0 - hr2;
@@ -123,18 +126,16 @@ HM_QUAL double log_pos(double u) {
     }
     const double s = (m - 1.0) * rcp<Ops>(m + 1.0);   // |s| <= 0.1716
     const double s2 = s * s;
-    // log m = 2 atanh s = 2 s (1 + s^2/3 + s^4/5 + ... + s^22/23)
-    double p = 1.0 / 23.0;
-    p = HM_FMAK(p, s2,1.0 / 21.0);
-    p = HM_FMAK(p, s2,1.0 / 19.0);
-    p = HM_FMAK(p, s2,1.0 / 17.0);
-    p = HM_FMAK(p, s2,1.0 / 15.0);
-    p = HM_FMAK(p, s2,1.0 / 13.0);
-    p = HM_FMAK(p, s2,1.0 / 11.0);
-    p = HM_FMAK(p, s2,1.0 / 9.0);
-    p = HM_FMAK(p, s2,1.0 / 7.0);
-    p = HM_FMAK(p, s2,1.0 / 5.0);
-    p = HM_FMAK(p, s2,1.0 / 3.0);
+    // log m = 2 atanh s = 2 s (1 + s^2 p(s^2)), p(t) ~ (atanh(s)/s - 1)/t =
+    // 1/3 + t/5 + ... (degree 7 in t, truncation error 1e-18 at |s| = 0.1716)
+    double p = 0.06543313073920051069031;
+    p = HM_FMAK(p, s2, 0.06634386569328442592349);
+    p = HM_FMAK(p, s2, 0.0769311985140732909423);
+    p = HM_FMAK(p, s2, 0.09090897820066453138014);
+    p = HM_FMAK(p, s2, 0.1111111119585695872093);
+    p = HM_FMAK(p, s2, 0.1428571428540180590539);
+    p = HM_FMAK(p, s2, 0.2000000000000043849136);
+    p = HM_FMAK(p, s2, 0.3333333333333333323247);
     const double t = s + s;
     const double lm = __builtin_fma(t * s2, p, t);
     const double de = (double)e;
