@@ -337,17 +337,31 @@ __device__ __forceinline__ double drift(const PassCtx &c, Flags<FX> fl, double z
     return z;
 }
 
-// RF, QE, SR and drift of one particle; returns dE' - dE_init via de_turn.
-template <uint32_t FX>
-__device__ __forceinline__ void advance(const PassCtx &c, Flags<FX> fl, double &z, double &e,
-                                        double r, double &de_turn) {
-    const double e_init = e;
+// The RF kick's energy change of a particle at z (rf_kick.jl:85-86); it
+// does not depend on the particle's energy.
+__device__ __forceinline__ double rf_term(const PassCtx &c, double z) {
     const double phi0 = ((-z) * c.rf) + c.phis;                    // rf_kick.jl:85
-    e = e + c.voltage * (dsin(phi0) - c.sin_phis);                 // rf_kick.jl:86
+    return c.voltage * (dsin(phi0) - c.sin_phis);                  // rf_kick.jl:86
+}
+
+// RF (its term precomputed by rf_term), QE, SR and drift of one particle;
+// returns dE' - dE_init via de_turn.
+template <uint32_t FX>
+__device__ __forceinline__ void advance_rest(const PassCtx &c, Flags<FX> fl, double &z, double &e,
+                                             double rf, double r, double &de_turn) {
+    const double e_init = e;
+    e = e + rf;                                                    // rf_kick.jl:86
     if (fl.has(F_QE)) e = e + c.exc * r;                           // quantum_excitation.jl:70
     if (fl.has(F_SR)) e = e * c.damp;                              // synchrotron_radiation.jl:55
     z = drift(c, fl, z, e);
     de_turn = e - e_init;
+}
+
+// RF, QE, SR and drift of one particle; returns dE' - dE_init via de_turn.
+template <uint32_t FX>
+__device__ __forceinline__ void advance(const PassCtx &c, Flags<FX> fl, double &z, double &e,
+                                        double r, double &de_turn) {
+    advance_rest(c, fl, z, e, rf_term(c, z), r, de_turn);
 }
 
 // Explicit address spaces.  Pointers read from a BunchDev are generic to the
@@ -420,7 +434,11 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
 // tile lt are lt*ppt + t + k*blockDim.x in ascending k, exactly as for a
 // tile taken alone: the loop keeps its one-pair prefetch running across the
 // tile boundaries and writes each tile's per-wave sums at the tile's end.
-template <uint32_t FX, bool LH>
+// KM, how the kick reads P: kFullWindow (the LDS window holds the whole table:
+// no global gathers), kGatherFirst or kPrefetchFirst (a partial window, the
+// particles outside it gather from global memory; see the loop body).
+enum KickMode { kFullWindow = 0, kGatherFirst = 1, kPrefetchFirst = 2 };
+template <uint32_t FX, bool LH, int KM>
 __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt1, int hb) {
     PassCtx c;
     load_ctx(B, c);
@@ -452,6 +470,10 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             zn = ldnt((const g_f64x2 *)(zp + 2 * p));
             en = ldnt((const g_f64x2 *)(ep + 2 * p));
         }
+        // the run's first pair completes here (s_waitcnt vmcnt(0)), so that in
+        // the loop the pair in hand is never taken for an outstanding load and
+        // the waits there count only the loads issued in the loop
+        __builtin_amdgcn_s_waitcnt(0x0F70);
     }
     for (int lt = lt0; lt < lt1; ++lt) {
         const int64_t p0 = (int64_t)lt * ppt;
@@ -460,30 +482,48 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
         double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
         for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) {
             const double2 zz = zn, ee = en;
+            // The previous turn's potential at (P[k], P[k+1]): read from the LDS
+            // window (clamped index, always valid), replaced by one 16-byte
+            // global gather for the particles outside a partial window.  With
+            // gather_first the gathers are issued before the next pair's
+            // prefetch and consumed after the RF and QE terms, so their wait
+            // leaves the prefetch in flight and that arithmetic covers their
+            // latency.  Otherwise the prefetch goes first.  The order of the
+            // loads changes no result.
+            constexpr bool gather = KM != kFullWindow;
+            constexpr bool gather_first = KM == kGatherFirst;
             const int64_t pn = p + blockDim.x;
-            if (pn < pf_run) {   // prefetch the next pair (the next tile's first at a tile end)
-                zn = ldnt((const g_f64x2 *)(zp + 2 * pn));
-                en = ldnt((const g_f64x2 *)(ep + 2 * pn));
-            }
-            // the previous turn's kick, interpolated between P[k] and P[k+1]:
-            // from the LDS window (clamped index, always valid), recomputed
-            // inside the branch from one 16-byte global gather for the few
-            // particles outside it.  The gather's wait stays in that branch, so
-            // the prefetch above remains in flight through the turn.
-            double kk0 = 0.0, kk1 = 0.0;
+            // the next pair (the next tile's first at a tile end); past the end
+            // of the run the current pair is read again and not used, so the
+            // load is unconditional and zn, en need no copy of their old value
+            auto prefetch = [&]() {
+                const int64_t pq = pn < pf_run ? pn : p;
+                zn = ldnt((const g_f64x2 *)(zp + 2 * pq));
+                en = ldnt((const g_f64x2 *)(ep + 2 * pq));
+            };
+            if (!gather_first) prefetch();
+            double w0 = 0.0, w1 = 0.0;
+            double2 q0 = make_double2(0.0, 0.0), q1 = make_double2(0.0, 0.0);
             if (kick2) {
                 int k0, k1;
-                double w0, w1;
                 interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
                 interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
                 const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
                 if (wpairs) {
                     const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
-                    kk0 = sP[i0] * (1.0 - w0) + sP[i0 + 1] * w0;
-                    kk1 = sP[i1] * (1.0 - w1) + sP[i1 + 1] * w1;
+                    q0 = make_double2(sP[i0], sP[i0 + 1]);
+                    q1 = make_double2(sP[i1], sP[i1 + 1]);
                 }
-                if (j0 >= wpairs) kk0 = gather_interp(P2 + k0, w0);
-                if (j1 >= wpairs) kk1 = gather_interp(P2 + k1, w1);
+                if (gather) {
+                    if (j0 >= wpairs) q0 = ld2(P2 + k0);
+                    if (j1 >= wpairs) q1 = ld2(P2 + k1);
+                }
+            }
+            if (gather_first) prefetch();
+            double rf0 = 0.0, rf1 = 0.0;
+            if (gather_first) {
+                rf0 = rf_term(c, zz.x);
+                rf1 = rf_term(c, zz.y);
             }
             double r0 = 0.0, r1 = 0.0;
             if (qe) {
@@ -497,8 +537,8 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             double z0 = zz.x, z1 = zz.y, e0 = ee.x, e1 = ee.y;
             if (c.pending_kick) {
                 if (kick2) {
-                    e0 = e0 - kk0;
-                    e1 = e1 - kk1;
+                    e0 = e0 - (q0.x * (1.0 - w0) + q0.y * w0);
+                    e1 = e1 - (q1.x * (1.0 - w1) + q1.y * w1);
                 } else {
                     e0 = e0 - c.P[0];
                     e1 = e1 - c.P[0];
@@ -509,8 +549,13 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
                 e1 = e1 - c.m_prev;
             }
             double d0, d1;
-            advance(c, fl, z0, e0, r0, d0);
-            advance(c, fl, z1, e1, r1, d1);
+            if (gather_first) {
+                advance_rest(c, fl, z0, e0, rf0, r0, d0);
+                advance_rest(c, fl, z1, e1, rf1, r1, d1);
+            } else {
+                advance(c, fl, z0, e0, r0, d0);
+                advance(c, fl, z1, e1, r1, d1);
+            }
             if (wake) {
                 bin_and_sum<LH>(c, ghist, z0, v);
                 bin_and_sum<LH>(c, ghist, z1, v);
@@ -602,10 +647,22 @@ __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bd
             staged = bi;
         }
         const int hb = pass_hist_bytes(B);
-        if (kLdsOnly || hb)
-            turn_pass_run<FX, true>(B, tile - B.tile_begin, te - B.tile_begin, hb);
-        else
-            turn_pass_run<FX, false>(B, tile - B.tile_begin, te - B.tile_begin, hb);
+        const int l0 = tile - B.tile_begin, l1 = te - B.tile_begin;
+        // a partial window of P: gather first in the pass with compile-time
+        // flags (only it has the registers), prefetch first in the generic one
+        constexpr int kPartial = FX != 0 ? kGatherFirst : kPrefetchFirst;
+        const bool full = B.win >= B.nbins;
+        if (kLdsOnly || hb) {
+            if (full)
+                turn_pass_run<FX, true, kFullWindow>(B, l0, l1, hb);
+            else
+                turn_pass_run<FX, true, kPartial>(B, l0, l1, hb);
+        } else {
+            if (full)
+                turn_pass_run<FX, false, kFullWindow>(B, l0, l1, hb);
+            else
+                turn_pass_run<FX, false, kPrefetchFirst>(B, l0, l1, hb);
+        }
         tile = te;
     }
     if (staged >= 0) flush_hist(bds[staged]);

@@ -42,7 +42,10 @@ def tune(nbins_list, n: int = 200_000, turns: int = 60, reps: int = 3) -> dict:
     out = {}
     for nb in nbins_list:
         best = None
-        for B in [64 << k for k in range(7) if (64 << k) <= min(maxb, max(nb, 64))]:
+        # blocks of at least nbins/128: the blocked convolution's work grows
+        # with the square of the block count
+        for B in [64 << k for k in range(7)
+                  if max(64, nb // 128) <= (64 << k) <= min(maxb, max(nb, 64))]:
             ms = []
             for r in range(reps):
                 b = H.DeviceBunch(n, dev)
@@ -64,14 +67,15 @@ def tune(nbins_list, n: int = 200_000, turns: int = 60, reps: int = 3) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--nbins", type=int, nargs="*",
-                    default=[256, 512, 1024, 2048, 4096, 8192, 16384])
+                    default=[256, 512, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072])
     ap.add_argument("--write", action="store_true", help="write tuning.json")
+    ap.add_argument("--out", default=TABLE, help="where --write puts the table")
     args = ap.parse_args()
     table = tune(args.nbins)
     print(json.dumps({"fft_block": table}))
     if args.write:
         import haissinskijulia_amd as H
-        with open(TABLE, "w") as f:
+        with open(args.out, "w") as f:
             json.dump({"device": H.MI355X.default().info()["arch"],
                        "fft_block": {str(k): v for k, v in sorted(table.items())}}, f, indent=1)
     return 0
