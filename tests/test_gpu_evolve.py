@@ -78,6 +78,50 @@ def test_device_philox_stream_matches_oracle(dev):
     _check(*_run_both(p, 8192, 512, normals=False, turn_offset=12345))
 
 
+@pytest.mark.parametrize("nbins,normals,turns", [(16384, False, 30), (16384, True, 30),
+                                                 (65536, False, 10)])
+def test_partial_window_kicks_match_oracle(dev, nbins, normals, turns):
+    # 16384 or 65536 bins over +-4 sigma_z0: the pass's LDS window of P (12160
+    # of 16384 entries, +-3 sigma_z0; or 20352 of 65536 with the histogram in
+    # global memory, +-1.2 sigma_z0) leaves the bunch's tails outside it, and
+    # those particles gather P from global memory.  With the device normals
+    # at 16384 bins the all-flags pass issues the gathers first; with injected
+    # normals, and at 65536 bins, the generic pass prefetches first.  At 65536
+    # bins a bin holds a third of a
+    # particle; the potential's bin-to-bin noise amplifies rounding
+    # differences faster, so that case runs 10 turns.
+    p = W.ring_3gev_lattice(n_turns=turns)
+    res = [W.RESONATOR_BB]
+    o = H.TrackingOptions(resonators=res, charge=5e-9, bin_half_width_sigmas=4.0)
+    okw = dict(resonators=res, charge=5e-9, bin_half_width_sigmas=4.0)
+    print("case", nbins, "normals" if normals else "philox", flush=True)
+    _check(*_run_both(p, 20_000, nbins, normals=normals, opts=o, oracle_kw=okw))
+
+
+@pytest.mark.parametrize("nbins", [8192, 16384])
+def test_pass_variants_agree_bitwise(dev, nbins):
+    # The all-flags pass (device Philox normals) against the generic pass fed
+    # the device's own normal stream (draw_normals, injected): the same
+    # arithmetic, so the same bits.  Bins over +-2 sigma_z0 leave one particle
+    # in seven outside the LDS window (+-1.5 sigma_z0) at 16384 bins: gather
+    # first against prefetch first.  At 8192 bins the window holds the table.
+    n, turns, seed = 20_000, 30, 101
+    p = to_sim_params(W.ring_3gev_lattice(n_turns=turns))
+    z0, e0 = O.generate_gaussian(1, 0.0, 4e-3, 0.0, 3e6, n)
+    kw = dict(resonators=[W.RESONATOR_BB], charge=5e-9, bin_half_width_sigmas=2.0, seed=seed)
+    out = []
+    for inject in (False, True):
+        o = H.TrackingOptions(**kw)
+        if inject:
+            o.normals = np.vstack([H.draw_normals(seed, t, 0, n, dev) for t in range(turns)])
+        part = H.Particles(z0.copy(), e0.copy())
+        r = H.longitudinal_evolve(part, p, H.create_simulation_buffers(n, nbins), dev, True, o)
+        out.append((part.coordinates.z.copy(), part.coordinates.ΔE.copy(), r))
+    (za, ea, ra), (zb, eb, rb) = out
+    assert np.array_equal(za, zb) and np.array_equal(ea, eb)
+    assert tuple(ra) == tuple(rb)
+
+
 def test_extensions_resonators_charge_half_width(dev):
     p = W.ring_3gev_lattice(n_turns=100)
     res = [W.RESONATOR_BB, W.RESONATOR_LOWQ]
