@@ -931,25 +931,17 @@ __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__res
 // E0 update support, moments, init, finalize
 // ---------------------------------------------------------------------------
 
-// The z loads of the interpolation sums and their tile order within a run.
-// Streaming loads leave the last-level cache to other data; the backward order
-// would start each run on the tiles the pass wrote last.
-constexpr bool kInterpStreaming = true;
-constexpr bool kInterpBackward = false;
-__device__ __forceinline__ double2 ldz(const g_f64x2 *a) {
-    return kInterpStreaming ? ldnt(a) : ld2(a);
-}
-
-// Sums over a run of tiles [lt0, lt1) of bunch B of the interpolated
-// potential at the post-drift z (the E0 update of a wake bunch), one sum per
-// tile and wave.  The run first stages B.iwin entries of P around the
+// interp_sum_run for tiles of at least four pairs per thread: per tile, groups
+// of four pairs per thread.  The sums over a run of tiles [lt0, lt1) of bunch
+// B of the interpolated potential at the post-drift z, one sum per tile and
+// wave.  The run first stages B.iwin entries of P around the
 // centroid in LDS (the whole table unless it exceeds the LDS); particles
 // outside the window read the pair table P2.  Each thread takes its pairs
 // of a tile in groups of four, in ascending order, with the next group's
 // streaming loads (the next tile's first group at a tile's end) issued before
 // this group's sums; the group loop has the same trip count in every lane.
 // A tile's sums therefore depend on the tile and the block size only.
-__device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
+__device__ inline void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1) {
     extern __shared__ double sP[];
     const int nb = B.nbins;
     const double *__restrict__ P = B.P;
@@ -974,7 +966,7 @@ __device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
     const bool ring = nb >= 2 && full >= 1;
     const int nrun = lt1 - lt0;
     // the j-th tile of the run, and its two-particle pairs
-    auto tile_of = [&](int j) { return kInterpBackward ? lt1 - 1 - j : lt0 + j; };
+    auto tile_of = [&](int j) { return lt0 + j; };
     auto lim_of = [&](int lt) {
         const int64_t p0 = (int64_t)lt * ppt;
         return (int)(min(min(p0 + ppt, B.npairs), full) - p0);
@@ -985,7 +977,7 @@ __device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int r = tid + u * ibs;
-            zn[u] = ldz((u < K && r < lim0) ? zpair + (int64_t)lt * ppt + r : zpair);
+            zn[u] = ldnt((u < K && r < lim0) ? zpair + (int64_t)lt * ppt + r : zpair);
         }
     }
     for (int j = 0; j < nrun; ++j) {
@@ -1015,7 +1007,7 @@ __device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
                 for (int u = 0; u < 4; ++u) {
                     const double2 zz = zn[u];
                     const int r = nr + u * ibs;
-                    zn[u] = ldz((u < nk && r < nlim) ? nbase + r : zpair);
+                    zn[u] = ldnt((u < nk && r < nlim) ? nbase + r : zpair);
                     if (k + u >= K || tid + (k + u) * ibs >= lim) continue;
                     int k0, k1;
                     double w0, w1;
@@ -1040,6 +1032,121 @@ __device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
         wave_sum<1>(v);
         if ((threadIdx.x & 63) == 0) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = v[0];
     }
+}
+
+// interp_sum_run for tiles of one to three pairs per thread.  The sums over a
+// run of tiles [lt0, lt1) of bunch B of the interpolated potential at the
+// post-drift z (the E0 update of a wake bunch), one sum per tile and wave.
+// The run first stages B.iwin entries of P around the
+// centroid in LDS (the whole table unless it exceeds the LDS); particles
+// outside the window read the pair table P2.  Each thread takes its pairs of
+// the run in order -- pair j in tile lt0 + j/K at offset threadIdx.x +
+// (j%K)*blockDim.x, K pairs per thread and tile -- through a ring of four
+// streaming loads in flight, so a tile of one pair per thread costs no
+// pipeline start.  A tile's sum is written when its last pair is added; it
+// depends on the tile and the block size only.
+__device__ inline void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) {
+    extern __shared__ double sP[];
+    const int nb = B.nbins;
+    const double *__restrict__ P = B.P;
+    const int w = B.iwin;
+    const int wlo = window_lo(B.sc->centre_bin, w, nb);
+    const unsigned wpairs = (unsigned)(w - 1);
+    __syncthreads();   // the previous run's window is no longer read
+    for (int k = threadIdx.x; k < w; k += blockDim.x) sP[k] = P[wlo + k];
+    __syncthreads();
+    const int64_t n = B.n, ppt = B.pairs_per_tile, full = n >> 1;
+    const int tid = (int)threadIdx.x, ibs = (int)blockDim.x;
+    const int K = (int)(ppt / ibs);   // pairs of each thread in a full tile
+    const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
+    const g_f64 *zp = (const g_f64 *)B.z;
+    const g_f64x2 *P2 = (const g_f64x2 *)B.P2;
+    const g_f64x2 *zpair = (const g_f64x2 *)zp;
+    // two-particle pairs of tile lt (32-bit offsets within a tile)
+    auto lim_of = [&](int lt) {
+        const int64_t p0 = (int64_t)lt * ppt;
+        return (int)(min(min(p0 + ppt, B.npairs), full) - p0);
+    };
+    // tile lt is complete: the single particle of an odd bunch (thread 0, the
+    // bunch's last tile), then the per-wave sums
+    auto flush = [&](double &v, int lt) {
+        const int64_t p0 = (int64_t)lt * ppt;
+        if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < min(p0 + ppt, B.npairs))
+            v += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
+        double s1[1] = {v};
+        wave_sum<1>(s1);
+        if ((threadIdx.x & 63) == 0) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = s1[0];
+        v = 0.0;
+    };
+    if (nb < 2 || full < 1) {   // a one-bin table, or no two-particle pair
+        for (int lt = lt0; lt < lt1; ++lt) {
+            double v = 0.0;
+            const int lim = lim_of(lt);
+            if (nb < 2)
+                for (int r = tid; r < lim; r += ibs) v += P[0] + P[0];
+            flush(v, lt);
+        }
+        return;
+    }
+    // The ring's loads are issued unconditionally (a lane without a next pair
+    // reads pair 0 and ignores it), so that every wait for a slot is the same
+    // count of younger loads.  The load and compute cursors are uniform.
+    int llt = lt0, lk = 0;
+    auto load_next = [&]() -> double2 {
+        const int r = tid + lk * ibs;
+        const bool ok = llt < lt1 && r < lim_of(llt);
+        const double2 x = ldnt(ok ? zpair + (int64_t)llt * ppt + r : zpair);
+        if (++lk == K) {
+            lk = 0;
+            ++llt;
+        }
+        return x;
+    };
+    double2 zn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) zn[u] = load_next();
+    const int J = (lt1 - lt0) * K;
+    int clt = lt0, ck = 0, clim = lim_of(lt0);
+    double v = 0.0;
+    for (int j = 0; j < J; j += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (j + u >= J) break;
+            const double2 zz = zn[u];
+            zn[u] = load_next();
+            if (tid + ck * ibs < clim) {
+                int k0, k1;
+                double w0, w1;
+                interp_index(zz.x, c0, c1, inv, nb, k0, w0);
+                interp_index(zz.y, c0, c1, inv, nb, k1, w1);
+                const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
+                const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
+                double t0 = sP[i0] * (1.0 - w0) + sP[i0 + 1] * w0;
+                double t1 = sP[i1] * (1.0 - w1) + sP[i1 + 1] * w1;
+                // outside the window: the pair table, with the wait in the branch
+                if (j0 >= wpairs) t0 = gather_interp(P2 + k0, w0);
+                if (j1 >= wpairs) t1 = gather_interp(P2 + k1, w1);
+                v += t0;
+                v += t1;
+            }
+            if (++ck == K) {   // the last pair of tile clt
+                flush(v, clt);
+                ck = 0;
+                ++clt;
+                if (clt < lt1) clim = lim_of(clt);
+            }
+        }
+    }
+}
+
+// Both forms add each thread's pairs of a tile in ascending order and reduce
+// them per wave: the same sums.  The per-tile groups cost less bookkeeping for
+// large tiles, the run-wide ring no pipeline start per tile for small ones.
+__device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
+    if (B.pairs_per_tile >= 4 * (int64_t)blockDim.x)
+        interp_sum_run_tiles(B, lt0, lt1);
+    else
+        interp_sum_run_flat(B, lt0, lt1);
 }
 
 // The interpolation sums of the wake + update_E0 bunches over a tile table:

@@ -87,9 +87,8 @@ def test_partial_window_kicks_match_oracle(dev, nbins, normals, turns):
     # those particles gather P from global memory.  With the device normals
     # at 16384 bins the all-flags pass issues the gathers first; with injected
     # normals, and at 65536 bins, the generic pass prefetches first.  At 65536
-    # bins a bin holds a third of a
-    # particle; the potential's bin-to-bin noise amplifies rounding
-    # differences faster, so that case runs 10 turns.
+    # bins a bin holds a third of a particle; the potential's bin-to-bin noise
+    # amplifies rounding differences faster, so that case runs 10 turns.
     p = W.ring_3gev_lattice(n_turns=turns)
     res = [W.RESONATOR_BB]
     o = H.TrackingOptions(resonators=res, charge=5e-9, bin_half_width_sigmas=4.0)
