@@ -221,6 +221,37 @@ __device__ __forceinline__ void wave_sum(double (&v)[NV]) {
     }
 }
 
+// One DPP lane move of a double (two 32-bit moves with the same control);
+// lanes whose row is not in ROWS receive 0.0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum of NV values over one wave in lane 63, by DPP lane moves (no LDS):
+// within each row of 16 lanes by quad permutes (lane ^ 1, lane ^ 2), the
+// half-row and the row mirror, which leave the row's sum in all of its lanes
+// (each step adds two equal partial sums, in either order the same bits);
+// then rows 1 and 3 add rows 0 and 2 (row_bcast:15) and row 3 adds row 1
+// (row_bcast:31): lane 63 holds (R3 + R2) + (R1 + R0).  A fixed order.
+template <int NV>
+__device__ __forceinline__ void wave_total(double (&v)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+        x = x + dpp_move<0xB1, 0xF>(x);    // quad_perm [1,0,3,2]
+        x = x + dpp_move<0x4E, 0xF>(x);    // quad_perm [2,3,0,1]
+        x = x + dpp_move<0x141, 0xF>(x);   // row_half_mirror
+        x = x + dpp_move<0x140, 0xF>(x);   // row_mirror
+        x = x + dpp_move<0x142, 0xA>(x);   // row_bcast:15 into rows 1, 3
+        x = x + dpp_move<0x143, 0x8>(x);   // row_bcast:31 into row 3
+        v[k] = x;
+    }
+}
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -586,8 +617,8 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
         if (!wake) { v[0] = 0.0; v[1] = 0.0; }
         // per-wave partial sums: no workgroup barrier per tile, so a wave never
         // waits for the others of its workgroup between tiles
-        wave_sum<4>(v);
-        if ((threadIdx.x & 63) == 0) {
+        wave_total<4>(v);
+        if ((threadIdx.x & 63) == 63) {
             double *o = B.part + 4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6));
             o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
         }
@@ -1029,7 +1060,7 @@ __device__ inline void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1)
         }
         if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < min(p0 + ppt, B.npairs))
             v[0] += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
-        wave_sum<1>(v);
+        wave_sum<1>(v);   // (the DPP reduction slows this loop's streaming)
         if ((threadIdx.x & 63) == 0) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = v[0];
     }
 }
@@ -1074,8 +1105,8 @@ __device__ inline void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) 
         if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < min(p0 + ppt, B.npairs))
             v += interp_pot(P, zp[n - 1], c0, c1, inv, nb);
         double s1[1] = {v};
-        wave_sum<1>(s1);
-        if ((threadIdx.x & 63) == 0) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = s1[0];
+        wave_total<1>(s1);
+        if ((threadIdx.x & 63) == 63) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = s1[0];
         v = 0.0;
     };
     if (nb < 2 || full < 1) {   // a one-bin table, or no two-particle pair
@@ -1140,7 +1171,9 @@ __device__ inline void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) 
 }
 
 // Both forms add each thread's pairs of a tile in ascending order and reduce
-// them per wave: the same sums.  The per-tile groups cost less bookkeeping for
+// them per wave in a fixed order (the large-tile form by a butterfly through
+// LDS permutes, the small-tile form by DPP moves).  Which form a bunch takes
+// depends on its size alone.  The per-tile groups cost less bookkeeping for
 // large tiles, the run-wide ring no pipeline start per tile for small ones.
 __device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
     if (B.pairs_per_tile >= 4 * (int64_t)blockDim.x)
