@@ -9,8 +9,10 @@
 //   lam[nbins]                  delta(c_i, 0.15) line-density kernel (wakefield.jl:219-221)
 //   What[nJ][F]   (complex)     FFT of the wake blocks, fixed for one evolve call
 //   X[nA][F]      (complex)     FFT of the packed line-density blocks of a turn
-//   part[ntiles][4]             per-tile sums  (sum z, sum z^2, sum dE'-dE_init, in-range count)
-//   ipart[ntiles]               per-tile sum of the interpolated potential (E0 update)
+//   part[ntiles][16][4]         per-wave sums of each tile (sum z, sum z^2,
+//                               sum dE'-dE_init, in-range count)
+//   ipart[ntiles][16]           per-wave sums of each tile of the interpolated
+//                               potential (E0 update)
 //   moments[ntiles][4]          per-tile sums of z, z^2, dE, dE^2 (initial / final spreads)
 //   sc                          TurnScalars of the current turn
 //   out[8]                      sigma_E0, sigma_z0, sigma_E, sigma_z, E0, ...
