@@ -19,7 +19,9 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import struct
 import sys
 import time
 
@@ -115,8 +117,12 @@ def bench_scan(dev, turns: int) -> dict:
     t_batch = time.perf_counter() - t0
     for b in bunches:
         b.close()
-    same = all((a.sigma_E, a.sigma_z, a.E0) == (b.sigma_E, b.sigma_z, b.E0)
-               for a, (_, b) in zip(rs, plain))
+    # bitwise: an unstable parameter point ends in NaN on both paths, and NaN
+    # never compares equal
+    def bits(r):
+        return struct.pack("<3d", r.sigma_E, r.sigma_z, r.E0)
+    same = all(bits(a) == bits(b) for a, (_, b) in zip(rs, plain))
+    nonfinite = sum(1 for a in rs if not all(math.isfinite(x) for x in (a.sigma_E, a.sigma_z, a.E0)))
     work = sum(s["n"] for s in specs) * turns
     lat_plain = [t for t, _ in plain]
     return {"workload": f"{len(specs)} bunches, N log-uniform [1e4, 4e6] (sum {work // turns:.3g}), "
@@ -127,16 +133,18 @@ def bench_scan(dev, turns: int) -> dict:
             "plain_particle_turns_per_s": work / lat_plain[-1],
             "p50_s": t_batch, "p99_s": t_batch,
             "plain_p50_s": _pct(lat_plain, 50), "plain_p99_s": _pct(lat_plain, 99),
-            "results_identical_to_plain": same}
+            "results_identical_to_plain": same,
+            "bunches_ending_non_finite": nonfinite}
 
 
-def bench_serve(dev, count: int, rate: float, turns_per_tick: int) -> dict:
+def bench_serve(dev, count: int, rate: float, turns_per_tick: int, n_max: float = 1e6,
+                turn_range=(100, 2000)) -> dict:
     """Continuous batching (hm_server) of a Poisson job stream in real time,
     against the plain path serving the same stream one job at a time (FIFO;
     its latencies follow from the measured service times and the arrivals)."""
     import haissinskijulia_amd as H
     from haissinskijulia_amd import workloads as W
-    jobs = W.serve_jobs(count, rate, n_range=(1e4, 1e6), turn_range=(100, 2000))
+    jobs = W.serve_jobs(count, rate, n_range=(1e4, n_max), turn_range=turn_range)
 
     def job(j):
         return H.Job(params=H.SimulationParameters.from_dict(
@@ -147,7 +155,11 @@ def bench_serve(dev, count: int, rate: float, turns_per_tick: int) -> dict:
 
     # plain path: service time of each job alone
     service = []
-    for j in jobs:
+    t_log = time.perf_counter()
+    for i, j in enumerate(jobs):
+        if time.perf_counter() - t_log > 30:
+            log(f"serving: plain {i}/{len(jobs)} jobs done")
+            t_log = time.perf_counter()
         b = H.DeviceBunch(j["n"], dev)
         t0 = time.perf_counter()
         b.generate(j["seed"], W.SIGMA_Z, W.SIGMA_E)
@@ -170,6 +182,9 @@ def bench_serve(dev, count: int, rate: float, turns_per_tick: int) -> dict:
             k += 1
         active = srv.tick(turns_per_tick)
         tnow = time.perf_counter() - t0
+        if time.perf_counter() - t_log > 30:
+            log(f"serving: served {len(lat)}/{len(jobs)} jobs, {k} submitted, t = {tnow:.0f} s")
+            t_log = time.perf_counter()
         for r in srv.poll():
             assert r.status == 0, r.status
             lat.append(tnow - submitted[r.job_id])
@@ -179,9 +194,10 @@ def bench_serve(dev, count: int, rate: float, turns_per_tick: int) -> dict:
     srv.close()
     work = sum(j["n"] * j["turns"] for j in jobs)
     span_plain = max(free, 0.0)
-    return {"workload": f"{count} jobs, Poisson {rate:g}/s, N log-uniform [1e4, 1e6], turns "
-                        f"log-uniform [100, 2000], 20% FD pairs (V x (1 -+ 0.5%), common random "
-                        f"numbers), 1024 bins, {turns_per_tick} turns per tick",
+    return {"workload": f"{count} jobs, Poisson {rate:g}/s, N log-uniform [1e4, {n_max:.0e}], "
+                        f"turns log-uniform [{turn_range[0]}, {turn_range[1]}], V U[2.5, 4.0] MV, "
+                        f"q U[0.5, 10] nC, 20% FD pairs (V x (1 -+ 0.5%), common random "
+                        f"numbers), 1024 bins, {turns_per_tick} turns per tick, 200 GB HBM budget",
             "p50_s": _pct(lat, 50), "p99_s": _pct(lat, 99),
             "particle_turns_per_s": work / span,
             "plain_p50_s": _pct(lat_plain, 50), "plain_p99_s": _pct(lat_plain, 99),
@@ -206,6 +222,10 @@ def main() -> int:
                     help="jobs of the serving run (server vs plain FIFO); 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=150.0, help="job arrivals per second")
     ap.add_argument("--serve-tick", type=int, default=25, help="turns per server tick")
+    ap.add_argument("--serve-n-max", type=float, default=1e6,
+                    help="largest particle count of a served job (the full stream: 1e7)")
+    ap.add_argument("--serve-turn-range", default="100,2000",
+                    help="turn counts of served jobs, lo,hi (the full stream: 1000,20000)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -287,7 +307,9 @@ def main() -> int:
             f"-> {out['scan']['speedup']:.1f}x, identical={out['scan']['results_identical_to_plain']}")
     if args.serve_jobs > 0:
         log(f"serving: {args.serve_jobs} jobs at {args.serve_rate:g}/s, plain then served ...")
-        out["serving"] = bench_serve(dev, args.serve_jobs, args.serve_rate, args.serve_tick)
+        lo, hi = (int(float(x)) for x in args.serve_turn_range.split(","))
+        out["serving"] = bench_serve(dev, args.serve_jobs, args.serve_rate, args.serve_tick,
+                                     args.serve_n_max, (lo, hi))
         sv = out["serving"]
         log(f"serving: p50 {sv['p50_s']:.3f} s p99 {sv['p99_s']:.3f} s; plain p50 "
             f"{sv['plain_p50_s']:.3f} s p99 {sv['plain_p99_s']:.3f} s")

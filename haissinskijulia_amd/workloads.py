@@ -115,11 +115,12 @@ def scan_bunches(count: int = 256, seed: int = 3) -> list[dict]:
 def serve_jobs(count: int, rate: float, n_range=(1e4, 1e7), turn_range=(1e3, 2e4),
                fd_fraction: float = 0.2, seed: int = 5) -> list[dict]:
     """A stream of tracking jobs on the 3 GeV ring: Poisson arrivals at `rate`
-    jobs/s, particle counts and turn counts log-uniform in the given ranges,
-    V_rf = 3.2 MV.  About `fd_fraction` of the jobs come as finite-difference
-    pairs (V_rf x (1 -+ 0.5%)) with common random numbers: both members have
-    the same arrival, particle seed and QE seed.  1024 bins over +-8 sigma_z,
-    1 nC, the broadband resonator."""
+    jobs/s; particle counts and turn counts log-uniform in the given ranges;
+    V_rf uniform in [2.5, 4.0] MV and charge uniform in [0.5, 10] nC.  About
+    `fd_fraction` of the jobs come as finite-difference pairs (V_rf x (1 -+
+    0.5%)) with common random numbers: both members have the same arrival,
+    particle seed, QE seed and charge.  1024 bins over +-8 sigma_z, the
+    broadband resonator."""
     import numpy as np
     rng = np.random.default_rng(seed)
     jobs: list[dict] = []
@@ -129,12 +130,14 @@ def serve_jobs(count: int, rate: float, n_range=(1e4, 1e7), turn_range=(1e3, 2e4
         n = int(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1]))))
         turns = int(math.exp(rng.uniform(math.log(turn_range[0]), math.log(turn_range[1]))))
         s = int(rng.integers(1, 2 ** 62))
-        base = dict(arrival=t, n=n, turns=turns, seed=s, V=3.2e6, charge=1e-9, nbins=1024,
+        V = float(rng.uniform(2.5e6, 4.0e6))
+        q = float(rng.uniform(0.5e-9, 10e-9))
+        base = dict(arrival=t, n=n, turns=turns, seed=s, V=V, charge=q, nbins=1024,
                     half_width=8.0, resonators=[RESONATOR_BB], pair=-1)
         if len(jobs) + 1 < count and rng.uniform() < fd_fraction / 2:
             k = len(jobs)
-            jobs.append(dict(base, V=3.2e6 * (1 - 0.005), pair=k + 1))
-            jobs.append(dict(base, V=3.2e6 * (1 + 0.005), pair=k))
+            jobs.append(dict(base, V=V * (1 - 0.005), pair=k + 1))
+            jobs.append(dict(base, V=V * (1 + 0.005), pair=k))
         else:
             jobs.append(base)
     return jobs
