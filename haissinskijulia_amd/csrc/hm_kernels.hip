@@ -118,8 +118,9 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        // three-input xor in one instruction (truth table 0x96 = a ^ b ^ c)
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -133,11 +134,26 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     x[0] = c0; x[1] = c1; x[2] = c2; x[3] = c3;
 }
 
+// sqrt(x) for the Box-Muller radius, 0 <= x <= 74 (u1 >= 2^-53): one
+// Goldschmidt step from the hardware reciprocal-square-root estimate and one
+// Newton correction, within an ulp of the rounded root.  x = 0 (u1 = 1) gives
+// 2^-500 instead of 0.
+__device__ __forceinline__ double bm_radius(double x) {
+    x = __builtin_fmax(x, 0x1p-1000);
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    const double d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
 // Box-Muller (1958) on one Philox block: u1 in (0,1], u2 in [0,1), 53 bits each
 __device__ __forceinline__ void box_muller(const uint32_t x[4], double &n0, double &n1) {
     const double u1 = ((double)(x[0] >> 5) * 67108864.0 + (double)(x[1] >> 6) + 1.0) * 0x1p-53;
     const double u2 = ((double)(x[2] >> 5) * 67108864.0 + (double)(x[3] >> 6)) * 0x1p-53;
-    const double r = sqrt(-2.0 * fm::log_pos<DevOps>(u1));
+    const double r = bm_radius(-2.0 * fm::log_pos<DevOps>(u1));
     double s, c;
     fm::sincos_reduced(kTwoPi * u2, s, c);
     n0 = r * c;
@@ -158,8 +174,8 @@ __device__ __forceinline__ void interp_index(double z, double c0, double c1, dou
     const double cl = (z > c1) ? c1 : ((z < c0) ? c0 : z);
     const double u = (cl - c0) * inv;
     const double fu = floor(u);
-    int kk = (fu == fu) ? (int)fu : 0;
-    k = kk < 0 ? 0 : (kk > nb - 2 ? nb - 2 : kk);
+    const int kk = (fu == fu) ? (int)fu : 0;
+    k = min(max(kk, 0), nb - 2);
     w = u - fu;
 }
 
@@ -533,8 +549,9 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
                 en = ldnt((const g_f64x2 *)(ep + 2 * pq));
             };
             if (!gather_first) prefetch();
-            double w0 = 0.0, w1 = 0.0;
-            double2 q0 = make_double2(0.0, 0.0), q1 = make_double2(0.0, 0.0);
+            // set and read only when kick2 (no per-pair clearing)
+            double w0, w1;
+            double2 q0, q1;
             if (kick2) {
                 int k0, k1;
                 interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
@@ -593,6 +610,9 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             }
             v[2] += d0;
             v[2] += d1;
+            // The prefetched pair is complete before this pair's stores are
+            // issued: a wait for a load behind a store would wait for the store.
+            if (KM == kFullWindow) __builtin_amdgcn_s_waitcnt(0x0F70);
             // streaming stores: the bunch is far larger than the caches, which
             // are left to the bin phase's tables
             stnt((g_f64x2 *)(zp + 2 * p), z0, z1);
