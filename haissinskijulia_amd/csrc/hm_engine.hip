@@ -244,8 +244,9 @@ static void balanced_ranges(const std::vector<int64_t> &w, int grid, std::vector
 Batch::~Batch() {
     drop_graph();
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
-    d_bd_.release(); d_tiles_.release(); d_list_.release();
+    d_bd_.release(); d_tiles_.release();
     for (auto &c : d_conv_) c.release();
+    d_red_.release();
     d_tmp_tiles_.release(); d_tmp_list_.release(); d_ranges_.release();
 }
 
@@ -269,7 +270,7 @@ hm_status Batch::upload_descriptors() {
 
 // launch tables of the active bunches
 hm_status Batch::build_tables() {
-    std::vector<int32_t> tiles, conv[3], list;
+    std::vector<int32_t> tiles, conv[3], red;
     pass_lds_ = 0;
     Fmax_ = 2;
     any_wake_ = any_wake_e0_ = any_reduce_ = false;
@@ -280,7 +281,10 @@ hm_status Batch::build_tables() {
         if (!used_[i] || !q.active) continue;
         q.d.tile_begin = (int32_t)tiles.size();
         for (int t = 0; t < q.d.ntiles; ++t) tiles.push_back((int32_t)i);
-        list.push_back((int32_t)i);
+        for (int r = 0; r < q.d.nred; ++r) {
+            red.push_back((int32_t)i);
+            red.push_back(r);
+        }
         if ((q.d.flags & kPhysMask) != kStdFlags) std_only_ = false;
         if (q.d.flags & (F_WAKE | F_E0)) any_reduce_ = true;
         if (q.d.flags & F_WAKE) {
@@ -305,7 +309,7 @@ hm_status Batch::build_tables() {
         }
     }
     ntiles_ = (int)tiles.size();
-    nlist_ = (int)list.size();
+    nred_ = (int)red.size() / 2;
     // Workgroup ranges: contiguous runs of the tile table with equal numbers
     // of pairs (tiles of different bunches differ in size), for the pass and
     // for the interpolation sums (which only count wake + update_E0 bunches).
@@ -329,7 +333,9 @@ hm_status Batch::build_tables() {
         HIP_TRY(hipMemcpyAsync(d_ranges_.p, ranges.data(), ranges.size() * 4, hipMemcpyHostToDevice,
                                s));
     HM_TRY(d_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
-    HM_TRY(d_list_.ensure(std::max<size_t>(list.size(), 1)));
+    HM_TRY(d_red_.ensure(std::max<size_t>(red.size(), 2)));
+    if (!red.empty())
+        HIP_TRY(hipMemcpyAsync(d_red_.p, red.data(), red.size() * 4, hipMemcpyHostToDevice, s));
     for (int k = 0; k < 3; ++k) {
         nconv_[k] = (int)conv[k].size() / 2;
         HM_TRY(d_conv_[k].ensure(std::max<size_t>(conv[k].size(), 2)));
@@ -339,8 +345,6 @@ hm_status Batch::build_tables() {
     }
     if (!tiles.empty())
         HIP_TRY(hipMemcpyAsync(d_tiles_.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, s));
-    if (!list.empty())
-        HIP_TRY(hipMemcpyAsync(d_list_.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
     HM_TRY(upload_descriptors());  // synchronises: the vectors above are still alive
     dirty_ = false;
     return HM_OK;
@@ -450,6 +454,12 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         HM_TRY(w.sc.ensure(1));
         HM_TRY(w.out.ensure(16));
         HIP_TRY(hipMemsetAsync(w.out.p, 0, 16 * sizeof(double), s));
+        d.nred = red_slices(d.ntiles);
+        HM_TRY(w.red.ensure((size_t)d.nred * 4));
+        HM_TRY(w.cnt.ensure(1));
+        HIP_TRY(hipMemsetAsync(w.cnt.p, 0, sizeof(uint32_t), s));   // set-up, never in a graph
+        d.red = w.red.p;
+        d.cnt = w.cnt.p;
         d.part = w.part.p;
         d.ipart = w.ipart.p;
         d.moments = w.moments.p;
@@ -615,7 +625,7 @@ hm_status Batch::one_turn() {
     HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, d_ranges_.p, pass_grid_, pass_lds_,
                              std_only_ ? 1 : 0, s));
     if (timing_) HIP_TRY(hipEventRecord(e1, s));
-    if (any_reduce_) HIP_TRY(launch_bin_reduce(d_bd_.p, d_list_.p, nlist_, s));
+    if (any_reduce_) HIP_TRY(launch_bin_reduce(d_bd_.p, d_red_.p, nred_, s));
     if (any_wake_) {
         HIP_TRY(launch_conv(CONV_FUSED, d_bd_.p, d_conv_[CONV_FUSED].p, nconv_[CONV_FUSED], Fmax_, s));
         HIP_TRY(launch_conv(CONV_FWD, d_bd_.p, d_conv_[CONV_FWD].p, nconv_[CONV_FWD], Fmax_, s));
@@ -625,7 +635,7 @@ hm_status Batch::one_turn() {
     if (any_wake_e0_)   // two workgroups per CU when both windows fit the LDS
         HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, d_ranges_.p + pass_grid_ + 1, interp_grid_,
                                   interp_lds_, s));
-    HIP_TRY(launch_finalize(d_bd_.p, d_list_.p, nlist_, s));
+    HIP_TRY(launch_finalize(d_bd_.p, d_red_.p, nred_, s));
     return HM_OK;
 }
 

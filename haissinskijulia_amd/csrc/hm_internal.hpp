@@ -62,15 +62,17 @@ struct Work {
     DevBuf<uint32_t> hist;
     DevBuf<double> P, P2, lam, What, X, Wtmp, part, ipart, moments, normals, out, stats, mon;
     DevBuf<TurnScalars> sc;
+    DevBuf<double> red;
+    DevBuf<uint32_t> cnt;
     void release() {
         hist.release(); P.release(); P2.release(); lam.release(); What.release(); X.release();
         Wtmp.release(); part.release(); ipart.release(); moments.release(); normals.release();
-        out.release(); stats.release(); mon.release(); sc.release();
+        out.release(); stats.release(); mon.release(); sc.release(); red.release(); cnt.release();
     }
     size_t bytes() const {
         return hist.bytes() + P.bytes() + P2.bytes() + lam.bytes() + What.bytes() + X.bytes() +
                Wtmp.bytes() + part.bytes() + ipart.bytes() + moments.bytes() + normals.bytes() +
-               out.bytes() + stats.bytes() + mon.bytes() + sc.bytes();
+               out.bytes() + stats.bytes() + mon.bytes() + sc.bytes() + red.bytes() + cnt.bytes();
     }
 };
 
@@ -167,13 +169,14 @@ class Batch {
     std::vector<RunBunch> rb_;
     std::vector<bool> used_;
     DevBuf<BunchDev> d_bd_;
-    DevBuf<int32_t> d_tiles_, d_list_, d_tmp_tiles_, d_tmp_list_;
+    DevBuf<int32_t> d_tiles_, d_tmp_tiles_, d_tmp_list_;
     // per-workgroup tile ranges of the pass and the interpolation sums,
     // [pass_grid_ + 1] then [interp_grid_ + 1] entries, balanced by pairs
     DevBuf<int32_t> d_ranges_;
     int pass_grid_ = 0, interp_grid_ = 0;
     DevBuf<int32_t> d_conv_[3];   // fused, forward, inverse convolution items
-    int ntiles_ = 0, nlist_ = 0;
+    DevBuf<int32_t> d_red_;       // (bunch, slice) items of the reductions
+    int ntiles_ = 0, nred_ = 0;
     int nconv_[3] = {0, 0, 0};
     size_t pass_lds_ = 0;
     int Fmax_ = 2;

@@ -8,9 +8,10 @@
 //                 drift (evolution.jl:330-347); it bins the new z into an
 //                 LDS-privatised histogram (utils.jl:351-401) and writes
 //                 deterministic per-wave partial sums of each tile.
-//   k_bin_reduce  one workgroup per bunch reduces those sums: sigma_z, the
-//                 wake current (evolution.jl:373-386), the binned count and
-//                 the sum of the energy changes.
+//   k_bin_reduce  one workgroup per slice of 1024 of those sums; the last
+//                 slice of a bunch to finish adds the slice sums in order:
+//                 sigma_z, the wake current (evolution.jl:373-386), the
+//                 binned count and the sum of the energy changes.
 //   k_conv_*      the line-density convolution with the wake
 //                 (wakefield.jl:186-319) by blocked LDS FFTs.
 //   k_interp_sum  (update_E0 with the wake) sum of the interpolated potential
@@ -743,50 +744,87 @@ __global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
 // since each block product g_a * W_j is real, has length 2B-1 <= F (no
 // wrap), and lands on output blocks a+j (first half) and a+j+1 (second half).
 
-// Sums over the n entries of a per-wave partial array, `NV` doubles per
-// entry (NV = 1 or 4).  Each thread adds entries threadIdx.x + k*blockDim.x
-// in ascending k, eight loads in flight; the workgroup sum follows in
-// block_sum's fixed order, so the result depends on n and the block size
-// only.
+// One slice of a per-bunch reduction: the sum of entries [r*kRedEntries,
+// (r+1)*kRedEntries) of a per-wave partial array of n entries, NV doubles
+// each (NV = 1 or 4).  Thread t adds entries r*kRedEntries + t + k*kRedThreads
+// in ascending k (all loads in flight at once), then block_sum adds the
+// threads in its fixed order: the slice sum depends on n and r only.  Valid
+// in thread 0.
 template <int NV>
-__device__ inline void sum_partials(const double *a, int n, double (&v)[NV]) {
-    const int st = (int)blockDim.x;
-    int t = (int)threadIdx.x;
-    if constexpr (NV == 4) {
-        const g_f64x2 *q = (const g_f64x2 *)a;
-        for (; t + 7 * st < n; t += 8 * st) {
-            double2 x[8], y[8];
+__device__ inline void slice_sum(const double *a, int n, int r, double (&v)[NV], double *s_red) {
+    const int e0 = r * kRedEntries + (int)threadIdx.x;
+    double x[kRedPer][NV];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                x[u] = ld2(q + 2 * (t + u * st));
-                y[u] = ld2(q + 2 * (t + u * st) + 1);
+    for (int k = 0; k < kRedPer; ++k) {
+        const int e = e0 + k * kRedThreads;
+        if constexpr (NV == 4) {
+            double2 lo = make_double2(0.0, 0.0), hi = make_double2(0.0, 0.0);
+            if (e < n) {
+                const g_f64x2 *q = (const g_f64x2 *)a + 2 * (int64_t)e;
+                lo = ld2(q);
+                hi = ld2(q + 1);
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                v[0] += x[u].x;
-                v[1] += x[u].y;
-                v[2] += y[u].x;
-                v[3] += y[u].y;
-            }
+            x[k][0] = lo.x;
+            x[k][1] = lo.y;
+            x[k][2] = hi.x;
+            x[k][3] = hi.y;
+        } else {
+            x[k][0] = e < n ? ((const g_f64 *)a)[e] : 0.0;
         }
-        for (; t < n; t += st) {
-            const double2 x = ld2(q + 2 * t), y = ld2(q + 2 * t + 1);
-            v[0] += x.x;
-            v[1] += x.y;
-            v[2] += y.x;
-            v[3] += y.y;
-        }
-    } else {
-        const g_f64 *q = (const g_f64 *)a;
-        for (; t + 7 * st < n; t += 8 * st) {
-            double x[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = q[t + u * st];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[0] += x[u];
-        }
-        for (; t < n; t += st) v[0] += q[t];
     }
+#pragma unroll
+    for (int k = 0; k < kRedPer; ++k)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] += x[k][j];
+    block_sum<NV>(v, s_red);
+}
+
+// Counts slice r of bunch B in, its NV sums (thread 0's v) first stored at
+// red[r].  Returns true in every thread of the workgroup that counts the
+// bunch's last slice in; there thread 0 receives in t the sums of all slices,
+// added in slice order, and the count is reset for the next reduction.  The
+// count's read-modify-write is acquire-release at device scope, so the last
+// slice sees every slice's stores whichever XCD wrote them; its threads read
+// the slice sums side by side (one round trip), thread 0 adds them.
+template <int NV>
+__device__ inline bool slices_done(const BunchDev &B, int r, const double (&v)[NV],
+                                   double (&t)[NV]) {
+    __shared__ int s_last;
+    __shared__ double s_slice[kRedThreads][NV];
+    g_f64 *red = (g_f64 *)B.red;
+    g_u32 *cnt = (g_u32 *)B.cnt;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            __hip_atomic_store(red + 4 * r + j, v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t before =
+            __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = before + 1 == (uint32_t)B.nred;
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+    for (int j = 0; j < NV; ++j) t[j] = 0.0;
+    for (int q0 = 0; q0 < B.nred; q0 += kRedThreads) {
+        const int q = q0 + (int)threadIdx.x;
+        if (q < B.nred) {
+            double x[NV];
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+                x[j] = __hip_atomic_load(red + 4 * q + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) s_slice[threadIdx.x][j] = x[j];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < kRedThreads && q0 + k < B.nred; ++k)
+#pragma unroll
+                for (int j = 0; j < NV; ++j) t[j] += s_slice[k][j];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
 }
 
 // Once per turn and bunch, after the pass: the pass's per-wave sums reduced
@@ -794,27 +832,28 @@ __device__ inline void sum_partials(const double *a, int n, double (&v)[NV]) {
 // bunch sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386),
 // the binned count, and the interpolation cell of the centroid (where the LDS
 // windows of P are placed; the placement changes no result); the turn
-// monitor's centroid, sigma_z and binned count.
-__global__ __launch_bounds__(kReduceThreads) void k_bin_reduce(const BunchDev *__restrict__ bds,
-                                                               const int32_t *__restrict__ list) {
+// monitor's centroid, sigma_z and binned count.  One workgroup per slice.
+__global__ __launch_bounds__(kRedThreads) void k_bin_reduce(const BunchDev *__restrict__ bds,
+                                                            const int32_t *__restrict__ items) {
     __shared__ double s_red[4 * 16];
-    const BunchDev &B = bds[list[blockIdx.x]];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int r = items[2 * blockIdx.x + 1];
     if (!(B.flags & (F_WAKE | F_E0))) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};   // sum z, sum z^2, sum (dE'-dE_init), binned count
-    sum_partials<4>(B.part, B.ntiles * kTileWaves, v);
-    block_sum<4>(v, s_red);
-    if (threadIdx.x != 0) return;
+    slice_sum<4>(B.part, B.ntiles * kTileWaves, r, v, s_red);
+    double t[4];
+    if (!slices_done<4>(B, r, v, t) || threadIdx.x != 0) return;
     TurnScalars &sc = *B.sc;
-    sc.sum_dE = v[2];
+    sc.sum_dE = t[2];
     if (!(B.flags & F_WAKE)) return;
-    const double sz = gstd_from_sums(v[0], v[1], (double)B.n);
-    const double mz = v[0] / (double)B.n;
+    const double sz = gstd_from_sums(t[0], t[1], (double)B.n);
+    const double mz = t[0] / (double)B.n;
     sc.sigma_z = sz;
     sc.current = (B.flags & F_CHARGE)
                      ? B.charge
                      : (((B.pf / sc.E0) / (kTwoPi * B.radius)) * sz) /
                            (sc.eta0 * (B.sigma_E0 * B.sigma_E0));
-    sc.n_h = v[3];
+    sc.n_h = t[3];
     const double kc = floor((mz - B.c_first) * B.inv_s);
     const double top = (double)(B.nbins - 1);
     sc.centre_bin = (kc == kc) ? (int)(kc < 0.0 ? 0.0 : (kc > top ? top : kc)) : 0;
@@ -822,7 +861,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_bin_reduce(const BunchDev *_
         double *m = B.mon + 4 * sc.turn;
         m[0] = mz;
         m[1] = sz;
-        m[3] = v[3];
+        m[3] = t[3];
     }
 }
 
@@ -1301,32 +1340,34 @@ __global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restri
     }
 }
 
-__global__ __launch_bounds__(kReduceThreads) void k_finalize(const BunchDev *__restrict__ bds,
-                                                             const int32_t *__restrict__ list) {
+// The E0 update (evolution.jl:419-429) from this turn's sums, and the scalars
+// of the next turn.  One workgroup per slice of the interpolation sums.
+__global__ __launch_bounds__(kRedThreads) void k_finalize(const BunchDev *__restrict__ bds,
+                                                          const int32_t *__restrict__ items) {
     __shared__ double s_red[16];
-    const BunchDev &B = bds[list[blockIdx.x]];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int r = items[2 * blockIdx.x + 1];
     const uint32_t fl = B.flags;
     double v[1] = {0.0};   // sum of the interpolated P (the sum of dE'-dE_init is in sc)
-    if ((fl & F_E0) && (fl & F_WAKE)) sum_partials<1>(B.ipart, B.ntiles * kTileWaves, v);
-    block_sum<1>(v, s_red);
-    if (threadIdx.x == 0) {
-        TurnScalars s = *B.sc;
-        double E0 = s.E0;
-        if (s.drift_skip) s.drift_skipped_count += 1;
-        if (fl & F_E0) {                                  // evolution.jl:419-429
-            const double mean = (s.sum_dE - v[0]) / (double)B.n;
-            E0 = E0 + mean;
-            s.m_prev = mean;
-            s.pending_m = 1;
-        } else {
-            s.pending_m = 0;
-        }
-        s.pending_kick = (fl & F_WAKE) ? 1 : 0;
-        if (B.mon) B.mon[4 * s.turn + 2] = E0;   // turn monitor: E0 after the turn
-        s.turn += 1;
-        turn_scalars(B, E0, s);
-        *B.sc = s;
+    if ((fl & F_E0) && (fl & F_WAKE)) slice_sum<1>(B.ipart, B.ntiles * kTileWaves, r, v, s_red);
+    double t[1];
+    if (!slices_done<1>(B, r, v, t) || threadIdx.x != 0) return;
+    TurnScalars s = *B.sc;
+    double E0 = s.E0;
+    if (s.drift_skip) s.drift_skipped_count += 1;
+    if (fl & F_E0) {                                  // evolution.jl:419-429
+        const double mean = (s.sum_dE - t[0]) / (double)B.n;
+        E0 = E0 + mean;
+        s.m_prev = mean;
+        s.pending_m = 1;
+    } else {
+        s.pending_m = 0;
     }
+    s.pending_kick = (fl & F_WAKE) ? 1 : 0;
+    if (B.mon) B.mon[4 * s.turn + 2] = E0;   // turn monitor: E0 after the turn
+    s.turn += 1;
+    turn_scalars(B, E0, s);
+    *B.sc = s;
 }
 
 __global__ void k_turn_scalars(const BunchDev *__restrict__ bd, double E0, TurnScalars *out) {
@@ -1672,15 +1713,15 @@ hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, const int32
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
-    if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(kReduceThreads), 0, s, bds, list);
+hipError_t launch_finalize(const BunchDev *bds, const int32_t *items, int nitems, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3(nitems), dim3(kRedThreads), 0, s, bds, items);
     return hipGetLastError();
 }
 
-hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
-    if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bin_reduce, dim3(nb), dim3(kReduceThreads), 0, s, bds, list);
+hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *items, int nitems, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_reduce, dim3(nitems), dim3(kRedThreads), 0, s, bds, items);
     return hipGetLastError();
 }
 

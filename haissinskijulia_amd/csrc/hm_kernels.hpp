@@ -13,13 +13,23 @@ constexpr int kPassThreads = 1024;     // turn-pass / moments / interp-sum block
 constexpr int kConvThreads = 256;      // convolution block
 constexpr int kConvInvThreads = 1024;  // phase-B convolution block (spectral sums in flight)
 constexpr int kFinalThreads = 256;     // per-bunch set-up and moment blocks
-constexpr int kReduceThreads = 1024;   // per-bunch reductions of the per-wave tile sums
+// Per-bunch reductions of the per-wave tile sums: each workgroup sums one
+// slice of kRedEntries entries (kRedPer per thread); the last slice of a bunch
+// to finish adds the slice sums in order.
+constexpr int kRedThreads = 256;
+constexpr int kRedPer = 4;
+constexpr int kRedEntries = kRedThreads * kRedPer;
 constexpr int kInterpThreads = 1024;   // LDS-staged interpolation sums
 // partial sums per tile are kept per wave of the 1024-thread pass and
 // interpolation-sum workgroups: part[ntiles][kTileWaves][4], ipart[ntiles][kTileWaves]
 constexpr int kTileWaves = 16;
 static_assert(kPassThreads == 64 * kTileWaves && kInterpThreads == 64 * kTileWaves,
               "per-wave tile sums assume 1024-thread workgroups");
+// slices of the reductions of a bunch with ntiles tiles
+inline int red_slices(int ntiles) {
+    const int e = (ntiles > 0 ? ntiles : 1) * kTileWaves;
+    return (e + kRedEntries - 1) / kRedEntries;
+}
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (128 KiB of complex FP64)
 constexpr int kAutoFFTBlock = 1024;    // default convolution block (F = 2048)
 
@@ -49,9 +59,11 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
 hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *ranges,
                              int grid, size_t lds_bytes, hipStream_t s);
 // per-bunch reduction of the pass's tile sums into the turn scalars (sum of
-// dE'-dE_init; sigma_z, current, binned count, centroid cell of a wake bunch)
-hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
-hipError_t launch_finalize(const BunchDev *bds, const int32_t *bunch_list, int nb, hipStream_t s);
+// dE'-dE_init; sigma_z, current, binned count, centroid cell of a wake bunch),
+// and of the interpolation sums into the E0 update.  items holds nitems
+// (bunch, slice) pairs, red_slices(ntiles) slices per bunch.
+hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *items, int nitems, hipStream_t s);
+hipError_t launch_finalize(const BunchDev *bds, const int32_t *items, int nitems, hipStream_t s);
 // FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
 hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,
                            hipStream_t s);

@@ -14,6 +14,9 @@
 //   ipart[ntiles][16]           per-wave sums of each tile of the interpolated
 //                               potential (E0 update)
 //   moments[ntiles][4]          per-tile sums of z, z^2, dE, dE^2 (initial / final spreads)
+//   red[nred][4]                slice sums of the per-bunch reductions, and cnt
+//                               (uint32), the slices of the running reduction
+//                               counted in so far (zero between reductions)
 //   sc                          TurnScalars of the current turn
 //   out[8]                      sigma_E0, sigma_z0, sigma_E, sigma_z, E0, ...
 #pragma once
@@ -96,6 +99,9 @@ struct BunchDev {
     int32_t win;          // entries of P the pass stages in LDS (0: all gathers global)
     int32_t iwin;         // entries of P the interpolation sums stage in LDS
     double *part, *ipart, *moments, *out;
+    double *red;          // [nred][4]
+    uint32_t *cnt;
+    int32_t nred, pad_red;
     TurnScalars *sc;
     double *mon;          // device [n_turns][4] turn monitor or null
 };
