@@ -324,6 +324,7 @@ hm_status Batch::build_tables() {
     const int per_cu = interp_lds_ + 1024 <= (size_t)ctx_->lds_max / 2 ? 2 : 1;
     pass_grid_ = std::min(ntiles_, ctx_->cus);
     interp_grid_ = any_wake_e0_ ? std::min(ntiles_, per_cu * ctx_->cus) : 0;
+    interp_deep_ = per_cu == 1;
     std::vector<int32_t> ranges;
     balanced_ranges(wpass, pass_grid_, ranges);
     balanced_ranges(winterp, interp_grid_, ranges);
@@ -634,7 +635,7 @@ hm_status Batch::one_turn() {
     if (timing_) HIP_TRY(hipEventRecord(e2, s));
     if (any_wake_e0_)   // two workgroups per CU when both windows fit the LDS
         HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, d_ranges_.p + pass_grid_ + 1, interp_grid_,
-                                  interp_lds_, s));
+                                  interp_lds_, interp_deep_, s));
     HIP_TRY(launch_finalize(d_bd_.p, d_red_.p, nred_, s));
     return HM_OK;
 }

@@ -174,6 +174,7 @@ class Batch {
     // [pass_grid_ + 1] then [interp_grid_ + 1] entries, balanced by pairs
     DevBuf<int32_t> d_ranges_;
     int pass_grid_ = 0, interp_grid_ = 0;
+    bool interp_deep_ = false;   // one interpolation workgroup per CU: deeper rings
     DevBuf<int32_t> d_conv_[3];   // fused, forward, inverse convolution items
     DevBuf<int32_t> d_red_;       // (bunch, slice) items of the reductions
     int ntiles_ = 0, nred_ = 0;

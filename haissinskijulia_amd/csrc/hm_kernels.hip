@@ -1021,17 +1021,18 @@ __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__res
 // E0 update support, moments, init, finalize
 // ---------------------------------------------------------------------------
 
-// interp_sum_run for tiles of at least four pairs per thread: per tile, groups
-// of four pairs per thread.  The sums over a run of tiles [lt0, lt1) of bunch
+// interp_sum_run for tiles of at least G pairs per thread (G = 4 or 8): per
+// tile, groups of G pairs per thread.  The sums over a run of tiles [lt0, lt1) of bunch
 // B of the interpolated potential at the post-drift z, one sum per tile and
 // wave.  The run first stages B.iwin entries of P around the
 // centroid in LDS (the whole table unless it exceeds the LDS); particles
 // outside the window read the pair table P2.  Each thread takes its pairs
-// of a tile in groups of four, in ascending order, with the next group's
+// of a tile in groups of G, in ascending order, with the next group's
 // streaming loads (the next tile's first group at a tile's end) issued before
 // this group's sums; the group loop has the same trip count in every lane.
-// A tile's sums therefore depend on the tile and the block size only.
-__device__ inline void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1) {
+// A tile's sums therefore depend on the tile and the block size only (not on G).
+template <int G>
+__device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1) {
     extern __shared__ double sP[];
     const int nb = B.nbins;
     const double *__restrict__ P = B.P;
@@ -1061,11 +1062,11 @@ __device__ inline void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1)
         const int64_t p0 = (int64_t)lt * ppt;
         return (int)(min(min(p0 + ppt, B.npairs), full) - p0);
     };
-    double2 zn[4];
+    double2 zn[G];
     if (ring) {
         const int lt = tile_of(0), lim0 = lim_of(lt);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < G; ++u) {
             const int r = tid + u * ibs;
             zn[u] = ldnt((u < K && r < lim0) ? zpair + (int64_t)lt * ppt + r : zpair);
         }
@@ -1076,14 +1077,14 @@ __device__ inline void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1)
         const int lim = lim_of(lt);   // two-particle pairs of this tile
         double v[1] = {0.0};
         if (ring) {
-            for (int k = 0; k < K; k += 4) {
+            for (int k = 0; k < K; k += G) {
                 // the next group: the rest of this tile, else the next tile's first
                 const g_f64x2 *nbase = zpair + p0;
                 int nr = 0, nlim = 0, nk = 0;
-                if (k + 4 < K) {
-                    nr = tid + (k + 4) * ibs;
+                if (k + G < K) {
+                    nr = tid + (k + G) * ibs;
                     nlim = lim;
-                    nk = K - (k + 4);
+                    nk = K - (k + G);
                 } else if (j + 1 < nrun) {
                     const int ltn = tile_of(j + 1);
                     nbase = zpair + (int64_t)ltn * ppt;
@@ -1091,10 +1092,10 @@ __device__ inline void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1)
                     nlim = lim_of(ltn);
                     nk = K;
                 }
-                // a ring of four slots: slot u's next load is issued as soon as
+                // a ring of G slots: slot u's next load is issued as soon as
                 // its current pair is taken
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < G; ++u) {
                     const double2 zz = zn[u];
                     const int r = nr + u * ibs;
                     zn[u] = ldnt((u < nk && r < nlim) ? nbase + r : zpair);
@@ -1135,7 +1136,7 @@ __device__ inline void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1)
 // streaming loads in flight, so a tile of one pair per thread costs no
 // pipeline start.  A tile's sum is written when its last pair is added; it
 // depends on the tile and the block size only.
-__device__ inline void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) {
+__device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) {
     extern __shared__ double sP[];
     const int nb = B.nbins;
     const double *__restrict__ P = B.P;
@@ -1234,28 +1235,47 @@ __device__ inline void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) 
 // LDS permutes, the small-tile form by DPP moves).  Which form a bunch takes
 // depends on its size alone.  The per-tile groups cost less bookkeeping for
 // large tiles, the run-wide ring no pipeline start per tile for small ones.
-__device__ inline void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
-    if (B.pairs_per_tile >= 4 * (int64_t)blockDim.x)
-        interp_sum_run_tiles(B, lt0, lt1);
+template <int G>
+__device__ __forceinline__ void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
+    if (G > 4 && B.pairs_per_tile >= G * (int64_t)blockDim.x)
+        interp_sum_run_tiles<G>(B, lt0, lt1);
+    else if (B.pairs_per_tile >= 4 * (int64_t)blockDim.x)
+        interp_sum_run_tiles<4>(B, lt0, lt1);
     else
         interp_sum_run_flat(B, lt0, lt1);
 }
 
 // The interpolation sums of the wake + update_E0 bunches over a tile table:
-// each workgroup walks its contiguous range of tiles, one run per bunch.  At
-// most 64 VGPRs, so that two workgroups fit a CU when both windows fit its LDS.
-__global__ __launch_bounds__(kInterpThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_interp_sum(
-    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
-    const int32_t *__restrict__ ranges) {
+// each workgroup walks its contiguous range of tiles, one run per bunch.
+template <int G>
+__device__ __forceinline__ void interp_sum_blocks(const BunchDev *__restrict__ bds,
+                                         const int32_t *__restrict__ tile_bunch,
+                                         const int32_t *__restrict__ ranges) {
     const int t0 = ranges[blockIdx.x];
     const int t1 = ranges[blockIdx.x + 1];
     for (int tile = t0; tile < t1;) {
         const BunchDev &B = bds[tile_bunch[tile]];
         const int te = min(t1, B.tile_begin + B.ntiles);
         if ((B.flags & F_WAKE) && (B.flags & F_E0))
-            interp_sum_run(B, tile - B.tile_begin, te - B.tile_begin);
+            interp_sum_run<G>(B, tile - B.tile_begin, te - B.tile_begin);
         tile = te;
     }
+}
+
+// At most 64 VGPRs, so that two workgroups fit a CU when both windows fit its
+// LDS; rings of four loads.
+__global__ __launch_bounds__(kInterpThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_interp_sum(
+    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
+    const int32_t *__restrict__ ranges) {
+    interp_sum_blocks<4>(bds, tile_bunch, ranges);
+}
+
+// One workgroup per CU (a window of P larger than half the LDS): the
+// registers of four waves per SIMD allow rings of eight loads.
+__global__ __launch_bounds__(kInterpThreads) void k_interp_sum_deep(
+    const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
+    const int32_t *__restrict__ ranges) {
+    interp_sum_blocks<8>(bds, tile_bunch, ranges);
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__restrict__ bds,
@@ -1644,7 +1664,7 @@ hipError_t configure_kernels(int lds) {
                          (const void *)k_conv_fused,
                          (const void *)k_conv_fwd, (const void *)k_conv_inv,
                          (const void *)k_fft_wake, (const void *)k_stage_hist,
-                         (const void *)k_interp_sum};
+                         (const void *)k_interp_sum, (const void *)k_interp_sum_deep};
     hipError_t last = hipSuccess;
     for (const void *f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -1706,10 +1726,14 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
 }
 
 hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, const int32_t *ranges,
-                             int grid, size_t lds_bytes, hipStream_t s) {
+                             int grid, size_t lds_bytes, bool deep, hipStream_t s) {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_interp_sum, dim3((unsigned)grid), dim3(kInterpThreads), lds_bytes, s, bds,
-                       tb, ranges);
+    if (deep)
+        hipLaunchKernelGGL(k_interp_sum_deep, dim3((unsigned)grid), dim3(kInterpThreads), lds_bytes,
+                           s, bds, tb, ranges);
+    else
+        hipLaunchKernelGGL(k_interp_sum, dim3((unsigned)grid), dim3(kInterpThreads), lds_bytes, s,
+                           bds, tb, ranges);
     return hipGetLastError();
 }
 
