@@ -78,6 +78,20 @@ def test_device_philox_stream_matches_oracle(dev):
     _check(*_run_both(p, 8192, 512, normals=False, turn_offset=12345))
 
 
+def test_large_bunch_matches_oracle(dev):
+    # 1.7e7 particles: tiles of 9216 pairs (nine per thread), so the
+    # interpolation sums run their eight-deep ring with a partial second group,
+    # and the per-bunch reductions span 15 slices of tile sums; at 16384 bins
+    # the interpolation window takes more than half the LDS (one workgroup per
+    # CU) and the pass's window is partial (gathers issued first).
+    p = W.ring_3gev_lattice(n_turns=2)
+    res = [W.RESONATOR_BB]
+    o = H.TrackingOptions(resonators=res, charge=5e-9)
+    okw = dict(resonators=res, charge=5e-9)
+    print("case 1.7e7 particles, 16384 bins, 2 turns", flush=True)
+    _check(*_run_both(p, 17_000_000, 16384, normals=False, opts=o, oracle_kw=okw))
+
+
 @pytest.mark.parametrize("nbins,normals,turns", [(16384, False, 30), (16384, True, 30),
                                                  (65536, False, 10)])
 def test_partial_window_kicks_match_oracle(dev, nbins, normals, turns):
