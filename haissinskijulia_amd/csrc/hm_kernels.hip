@@ -169,10 +169,13 @@ __device__ __forceinline__ void qe_pair_normals(uint64_t seed, uint64_t p, uint3
     box_muller(x, n0, n1);
 }
 
-// zero_alloc_interpolation! index and weight (wakefield.jl:38-42,64-78) for nb >= 2
+// zero_alloc_interpolation! index and weight (wakefield.jl:38-42,64-78) for nb >= 2.
+// The clamp of z to [c0, c1] is one v_max_f64 and one v_min_f64; a NaN z
+// (which the reference rejects) takes cell 0 with weight 0, and its particle
+// still ends NaN through the RF kick.
 __device__ __forceinline__ void interp_index(double z, double c0, double c1, double inv, int nb,
                                              int &k, double &w) {
-    const double cl = (z > c1) ? c1 : ((z < c0) ? c0 : z);
+    const double cl = __builtin_fmin(__builtin_fmax(z, c0), c1);
     const double u = (cl - c0) * inv;
     const double fu = floor(u);
     const int kk = (fu == fu) ? (int)fu : 0;
@@ -196,11 +199,12 @@ __device__ __forceinline__ uint32_t *hist_of(const BunchDev &B, int64_t turn) {
 }
 
 // histogram bin of calculate_local_histogram! (utils.jl:387-397); -1 = not counted
+// (z >= e0 makes z - e0, u and b non-negative: only z = e1 needs the clamp)
 __device__ __forceinline__ int hist_bin(double z, double e0, double e1, double inv, int nb) {
     if (!((z >= e0) & (z <= e1))) return -1;
     const double u = (z - e0) * inv;
     const int b = (int)floor(u);
-    return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+    return min(b, nb - 1);
 }
 
 // Deterministic block sum of NV values: wave butterflies, then the waves in
@@ -558,7 +562,7 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
                 interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
                 interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
                 const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
-                if (wpairs) {
+                if (KM == kFullWindow || wpairs) {   // a full window has wpairs >= 1
                     const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
                     q0 = make_double2(sP[i0], sP[i0 + 1]);
                     q1 = make_double2(sP[i1], sP[i1 + 1]);
