@@ -270,12 +270,25 @@ hm_status Batch::upload_descriptors() {
 
 // launch tables of the active bunches
 hm_status Batch::build_tables() {
-    std::vector<int32_t> tiles, conv[3], red;
+    std::vector<int32_t> tiles, conv[kConvKinds], red;
     pass_lds_ = 0;
+    spec_lds_ = 0;
     Fmax_ = 2;
     any_wake_ = any_wake_e0_ = any_reduce_ = false;
     std_only_ = true;
     interp_lds_ = 0;
+    // The inverse phase in two kernels (spectral sums per frequency chunk, then
+    // the inverse FFTs) pays an extra launch; it is used when the one-kernel
+    // form would re-read more than kSpecMinBytes of spectra (nA^2 F / 2 complex
+    // pairs per bunch), e.g. for many bunches.  Both forms give the same bits.
+    constexpr double kSpecMinBytes = 24e6;
+    double reread = 0.0;
+    for (size_t i = 0; i < rb_.size(); ++i) {
+        const RunBunch &q = rb_[i];
+        if (used_[i] && q.active && (q.d.flags & F_WAKE) && q.d.nJ > 1)
+            reread += (double)q.d.nA * (double)q.d.nA * (double)q.d.F * 16.0;
+    }
+    const bool use_spec = reread > kSpecMinBytes;
     for (size_t i = 0; i < rb_.size(); ++i) {
         RunBunch &q = rb_[i];
         if (!used_[i] || !q.active) continue;
@@ -293,6 +306,12 @@ hm_status Batch::build_tables() {
                 any_wake_e0_ = true;
                 interp_lds_ = std::max(interp_lds_, (size_t)q.d.iwin * 8);
             }
+            // the inverse phase: spectral sums per frequency chunk, then one
+            // inverse FFT per output block (CONV_SPEC, CONV_OUT), or both per
+            // output block (CONV_INV: a small re-read volume, or a chunk's
+            // spectra larger than the LDS)
+            const bool spec = use_spec && q.d.Y;
+            const int binv = spec ? CONV_OUT : CONV_INV;
             for (int o = 0; o < q.d.nA; ++o) {
                 if (q.d.nJ <= 1) {
                     conv[CONV_FUSED].push_back((int32_t)i);
@@ -300,9 +319,16 @@ hm_status Batch::build_tables() {
                 } else {
                     conv[CONV_FWD].push_back((int32_t)i);
                     conv[CONV_FWD].push_back(o);
-                    conv[CONV_INV].push_back((int32_t)i);
-                    conv[CONV_INV].push_back(o);
+                    conv[binv].push_back((int32_t)i);
+                    conv[binv].push_back(o);
                 }
+            }
+            if (q.d.nJ > 1 && spec) {
+                for (int c = 0; c * spec_chunk(q.d.nA, q.d.F) < q.d.F; ++c) {
+                    conv[CONV_SPEC].push_back((int32_t)i);
+                    conv[CONV_SPEC].push_back(c);
+                }
+                spec_lds_ = std::max(spec_lds_, conv_spec_lds(q.d.nA, q.d.nJ, q.d.F));
             }
             Fmax_ = std::max(Fmax_, q.d.F);
             pass_lds_ = std::max(pass_lds_, pass_hist_bytes(q.d) + (size_t)q.d.win * 8);
@@ -337,7 +363,7 @@ hm_status Batch::build_tables() {
     HM_TRY(d_red_.ensure(std::max<size_t>(red.size(), 2)));
     if (!red.empty())
         HIP_TRY(hipMemcpyAsync(d_red_.p, red.data(), red.size() * 4, hipMemcpyHostToDevice, s));
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < kConvKinds; ++k) {
         nconv_[k] = (int)conv[k].size() / 2;
         HM_TRY(d_conv_[k].ensure(std::max<size_t>(conv[k].size(), 2)));
         if (!conv[k].empty())
@@ -567,9 +593,14 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         HM_TRY(w.lam.ensure((size_t)nb));
         HM_TRY(w.Wtmp.ensure((size_t)nb));
         HM_TRY(w.What.ensure((size_t)std::max(d.nJ, 1) * (size_t)d.F * 2));
+        d.Y = nullptr;
         if (d.nJ > 1) {
             HM_TRY(w.X.ensure((size_t)d.nA * (size_t)d.F * 2));
             d.X = w.X.p;
+            if (conv_spec_fits(d.nA, d.nJ, d.F, ctx_->lds_max)) {
+                HM_TRY(w.Y.ensure((size_t)d.nA * (size_t)d.F * 2));
+                d.Y = w.Y.p;
+            }
         }
         HIP_TRY(hipMemcpyAsync(w.lam.p, lam.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(w.Wtmp.p, W.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
@@ -630,6 +661,9 @@ hm_status Batch::one_turn() {
     if (any_wake_) {
         HIP_TRY(launch_conv(CONV_FUSED, d_bd_.p, d_conv_[CONV_FUSED].p, nconv_[CONV_FUSED], Fmax_, s));
         HIP_TRY(launch_conv(CONV_FWD, d_bd_.p, d_conv_[CONV_FWD].p, nconv_[CONV_FWD], Fmax_, s));
+        HIP_TRY(launch_conv(CONV_SPEC, d_bd_.p, d_conv_[CONV_SPEC].p, nconv_[CONV_SPEC],
+                            (int)spec_lds_, s));
+        HIP_TRY(launch_conv(CONV_OUT, d_bd_.p, d_conv_[CONV_OUT].p, nconv_[CONV_OUT], Fmax_, s));
         HIP_TRY(launch_conv(CONV_INV, d_bd_.p, d_conv_[CONV_INV].p, nconv_[CONV_INV], Fmax_, s));
     }
     if (timing_) HIP_TRY(hipEventRecord(e2, s));

@@ -62,17 +62,19 @@ struct Work {
     DevBuf<uint32_t> hist;
     DevBuf<double> P, P2, lam, What, X, Wtmp, part, ipart, moments, normals, out, stats, mon;
     DevBuf<TurnScalars> sc;
-    DevBuf<double> red;
+    DevBuf<double> red, Y;
     DevBuf<uint32_t> cnt;
     void release() {
         hist.release(); P.release(); P2.release(); lam.release(); What.release(); X.release();
         Wtmp.release(); part.release(); ipart.release(); moments.release(); normals.release();
         out.release(); stats.release(); mon.release(); sc.release(); red.release(); cnt.release();
+        Y.release();
     }
     size_t bytes() const {
         return hist.bytes() + P.bytes() + P2.bytes() + lam.bytes() + What.bytes() + X.bytes() +
                Wtmp.bytes() + part.bytes() + ipart.bytes() + moments.bytes() + normals.bytes() +
-               out.bytes() + stats.bytes() + mon.bytes() + sc.bytes() + red.bytes() + cnt.bytes();
+               out.bytes() + stats.bytes() + mon.bytes() + sc.bytes() + red.bytes() + cnt.bytes() +
+               Y.bytes();
     }
 };
 
@@ -175,10 +177,11 @@ class Batch {
     DevBuf<int32_t> d_ranges_;
     int pass_grid_ = 0, interp_grid_ = 0;
     bool interp_deep_ = false;   // one interpolation workgroup per CU: deeper rings
-    DevBuf<int32_t> d_conv_[3];   // fused, forward, inverse convolution items
+    DevBuf<int32_t> d_conv_[kConvKinds];   // convolution items of each ConvKind
     DevBuf<int32_t> d_red_;       // (bunch, slice) items of the reductions
     int ntiles_ = 0, nred_ = 0;
-    int nconv_[3] = {0, 0, 0};
+    int nconv_[kConvKinds] = {0, 0, 0, 0, 0};
+    size_t spec_lds_ = 0;   // LDS of the spectral-sum workgroups
     size_t pass_lds_ = 0;
     int Fmax_ = 2;
     bool any_wake_ = false, any_wake_e0_ = false;   // some bunch with wake / wake + update_E0

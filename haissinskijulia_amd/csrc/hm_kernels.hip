@@ -795,6 +795,11 @@ __device__ inline bool slices_done(const BunchDev &B, int r, const double (&v)[N
                                    double (&t)[NV]) {
     __shared__ int s_last;
     __shared__ double s_slice[kRedThreads][NV];
+    if (B.nred == 1) {   // one slice: nothing to count or to exchange
+#pragma unroll
+        for (int j = 0; j < NV; ++j) t[j] = v[j];
+        return true;
+    }
     g_f64 *red = (g_f64 *)B.red;
     g_u32 *cnt = (g_u32 *)B.cnt;
     if (threadIdx.x == 0) {
@@ -963,8 +968,69 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const BunchDev *__res
     for (int k = threadIdx.x; k < B.F; k += blockDim.x) X[k] = sX[k];
 }
 
-// phase B: Y_o = sum_j X_{o-j} FFT(W_j), inverse FFT, P block o.  The
-// products are summed in j order; four j at a time have their loads in flight.
+// phase B1: Y_o[k] = sum_{j <= min(o, nJ-1)} X_{o-j}[k] FFT(W_j)[k] for every
+// output block o, one chunk of spec_chunk(nA, F) frequencies of one bunch per
+// workgroup.  The chunk's columns of X and of the wake spectra are staged in
+// LDS once (every X_a and W_j is read from memory once per turn, instead of
+// once per output block as in k_conv_inv); a thread takes one (frequency,
+// output block) pair.  The products are added in ascending j from zero, as in
+// k_conv_inv, so Y_o and the potential are the same bits on either path.
+__global__ __launch_bounds__(kConvThreads) void k_conv_spec(const BunchDev *__restrict__ bds,
+                                                            const int32_t *__restrict__ items) {
+    extern __shared__ double2 sS[];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int c = items[2 * blockIdx.x + 1];
+    if (B.sc->n_h == 0.0) return;   // k_conv_out writes a zero potential
+    const int F = B.F, nA = B.nA, nJ = B.nJ;
+    const int kc = spec_chunk(nA, F), k0 = c * kc;
+    double2 *sXc = sS;               // [nA][kc]
+    double2 *sWc = sS + nA * kc;     // [nJ][kc]
+    const double2 *X = reinterpret_cast<const double2 *>(B.X);
+    const double2 *What = reinterpret_cast<const double2 *>(B.What);
+    for (int i = threadIdx.x; i < nA * kc; i += blockDim.x) {
+        const int a = i / kc, kk = i - a * kc;
+        sXc[i] = X[(int64_t)a * F + k0 + kk];
+    }
+    for (int i = threadIdx.x; i < nJ * kc; i += blockDim.x) {
+        const int j = i / kc, kk = i - j * kc;
+        sWc[i] = What[(int64_t)j * F + k0 + kk];
+    }
+    __syncthreads();
+    double2 *Y = reinterpret_cast<double2 *>(B.Y);
+    for (int i = threadIdx.x; i < nA * kc; i += blockDim.x) {
+        const int o = i / kc, kk = i - o * kc;
+        double2 acc = make_double2(0.0, 0.0);
+        const int jmax = min(o, nJ - 1);
+#pragma unroll 4
+        for (int j = 0; j <= jmax; ++j) {
+            const double2 t = cmul(sXc[(o - j) * kc + kk], sWc[j * kc + kk]);
+            acc = make_double2(acc.x + t.x, acc.y + t.y);
+        }
+        Y[(int64_t)o * F + k0 + kk] = acc;
+    }
+}
+
+// phase B2: inverse FFT of Y_o, P block o
+__global__ __launch_bounds__(kConvThreads) void k_conv_out(const BunchDev *__restrict__ bds,
+                                                           const int32_t *__restrict__ items) {
+    extern __shared__ double2 sX[];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int o = items[2 * blockIdx.x + 1];
+    const double nh = B.sc->n_h, cur = B.sc->current;
+    if (nh == 0.0) {
+        zero_block(B, o);
+        return;
+    }
+    const int F = B.F;
+    const double2 *Y = reinterpret_cast<const double2 *>(B.Y) + (int64_t)o * F;
+    for (int k = threadIdx.x; k < F; k += blockDim.x) sX[k] = Y[k];
+    fft_dif_inverse(sX, F, B.logF, reinterpret_cast<const double2 *>(B.tw));
+    store_block(B, sX, o, cur);
+}
+
+// phase B (where the spectra of a chunk exceed the LDS): Y_o = sum_j
+// X_{o-j} FFT(W_j), inverse FFT, P block o.  The products are summed in j
+// order; four j at a time have their loads in flight.
 __global__ __launch_bounds__(kConvInvThreads) void k_conv_inv(const BunchDev *__restrict__ bds,
                                                               const int32_t *__restrict__ items) {
     extern __shared__ double2 sX[];
@@ -1667,6 +1733,7 @@ hipError_t configure_kernels(int lds) {
     const void *fns[] = {(const void *)k_turn_pass, (const void *)k_turn_pass_std,
                          (const void *)k_conv_fused,
                          (const void *)k_conv_fwd, (const void *)k_conv_inv,
+                         (const void *)k_conv_spec, (const void *)k_conv_out,
                          (const void *)k_fft_wake, (const void *)k_stage_hist,
                          (const void *)k_interp_sum, (const void *)k_interp_sum_deep};
     hipError_t last = hipSuccess;
@@ -1720,12 +1787,17 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
                        hipStream_t s) {
     if (nitems <= 0) return hipSuccess;
     const size_t lds = (size_t)Fmax * 16;
-    if (kind == 0)
+    if (kind == CONV_FUSED)
         hipLaunchKernelGGL(k_conv_fused, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
-    else if (kind == 1)
+    else if (kind == CONV_FWD)
         hipLaunchKernelGGL(k_conv_fwd, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
-    else
+    else if (kind == CONV_INV)
         hipLaunchKernelGGL(k_conv_inv, dim3(nitems), dim3(kConvInvThreads), lds, s, bds, items);
+    else if (kind == CONV_SPEC)
+        hipLaunchKernelGGL(k_conv_spec, dim3(nitems), dim3(kConvThreads), (size_t)Fmax, s, bds,
+                           items);
+    else
+        hipLaunchKernelGGL(k_conv_out, dim3(nitems), dim3(kConvThreads), lds, s, bds, items);
     return hipGetLastError();
 }
 

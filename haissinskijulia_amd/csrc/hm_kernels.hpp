@@ -33,7 +33,24 @@ inline int red_slices(int ntiles) {
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (128 KiB of complex FP64)
 constexpr int kAutoFFTBlock = 1024;    // default convolution block (F = 2048)
 
-enum ConvKind { CONV_FUSED = 0, CONV_FWD = 1, CONV_INV = 2 };
+// CONV_SPEC + CONV_OUT replace CONV_INV where the spectra of a frequency
+// chunk fit the LDS (conv_spec_fits)
+enum ConvKind { CONV_FUSED = 0, CONV_FWD = 1, CONV_INV = 2, CONV_SPEC = 3, CONV_OUT = 4 };
+constexpr int kConvKinds = 5;
+// Frequencies per CONV_SPEC workgroup: one (frequency, output block) pair per
+// thread of a 256-thread workgroup, 1 to 16 frequencies (F and the result
+// are powers of two, so the chunks tile F).
+__host__ __device__ inline int spec_chunk(int nA, int F) {
+    int c = 16;
+    while (c > 1 && (c * nA > 256 || c > F)) c >>= 1;
+    return c;
+}
+inline size_t conv_spec_lds(int nA, int nJ, int F) {
+    return (size_t)(nA + nJ) * (size_t)spec_chunk(nA, F) * 16;
+}
+inline bool conv_spec_fits(int nA, int nJ, int F, int lds_max) {
+    return conv_spec_lds(nA, nJ, F) + 1024 <= (size_t)lds_max;
+}
 
 // allow the dynamic-LDS kernels the device's LDS per block (best effort)
 hipError_t configure_kernels(int max_dynamic_lds);
@@ -51,7 +68,9 @@ hipError_t launch_init(const BunchDev *bds, const int32_t *bunch_list, int nb, h
 // potential window; std_only: every tile's bunch has (flags & kPhysMask) == kStdFlags
 hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *ranges,
                             int grid, size_t lds_bytes, int std_only, hipStream_t s);
-// items are (bunch, block) pairs; Fmax is the largest FFT length among them
+// items are (bunch, block) pairs, for CONV_SPEC (bunch, frequency chunk)
+// pairs; Fmax is the largest FFT length among them, for CONV_SPEC the largest
+// conv_spec_lds of their bunches in bytes
 hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int nitems, int Fmax,
                        hipStream_t s);
 // interpolation sums of the wake + update_E0 bunches: `grid` workgroups over

@@ -9,6 +9,8 @@
 //   lam[nbins]                  delta(c_i, 0.15) line-density kernel (wakefield.jl:219-221)
 //   What[nJ][F]   (complex)     FFT of the wake blocks, fixed for one evolve call
 //   X[nA][F]      (complex)     FFT of the packed line-density blocks of a turn
+//   Y[nA][F]      (complex)     their products with the wake spectra, summed
+//                               per output block (before its inverse FFT)
 //   part[ntiles][16][4]         per-wave sums of each tile (sum z, sum z^2,
 //                               sum dE'-dE_init, in-range count)
 //   ipart[ntiles][16]           per-wave sums of each tile of the interpolated
@@ -95,6 +97,7 @@ struct BunchDev {
     const double *What;   // interleaved complex [nJ][F][2]
     const double *tw;     // interleaved complex [F/2][2], exp(-2 pi i k / F)
     double *X;            // interleaved complex [nA][F][2]: FFT(g_a + i g_{a-1}) (nJ > 1)
+    double *Y;            // interleaved complex [nA][F][2]: sum_j X_{o-j} FFT(W_j), or null
     int32_t B, F, logF, nA, nJ;
     int32_t win;          // entries of P the pass stages in LDS (0: all gathers global)
     int32_t iwin;         // entries of P the interpolation sums stage in LDS

@@ -66,6 +66,30 @@ def test_batch_equals_each_bunch_alone_bitwise(turns):
         b.close()
 
 
+def test_spectral_sum_convolution_equals_block_convolution_bitwise():
+    # A bunch of 4096 bins alone re-reads ~4 MB of spectra per turn and takes
+    # the one-kernel inverse phase (k_conv_inv); eight of them in a batch take
+    # the spectral sums per frequency chunk (k_conv_spec, k_conv_out).  Both
+    # add the same products in the same order: the same bits.
+    specs = [dict(n=20_000 + 3_000 * i, nbins=4096, V=2.6e6 + 0.2e6 * i, charge=(1 + i) * 1e-9,
+                  seed=40 + i, flags=(True, True, True, True, True)) for i in range(8)]
+    alone = []
+    for s in specs:
+        b = _bunch(s)
+        r = b.evolve(_params(30, s["V"]), s["nbins"], _opts(s))
+        alone.append((r, *b.download()))
+        b.close()
+    bunches = [_bunch(s) for s in specs]
+    rs = H.batch_evolve(bunches, [s["nbins"] for s in specs], [_params(30, s["V"]) for s in specs],
+                        [_opts(s) for s in specs])
+    for s, b, r, (ra, za, ea) in zip(specs, bunches, rs, alone):
+        z, e = b.download()
+        print(f"n={s['n']} sigma_E={r.sigma_E:.6e} E0={r.E0:.6e}", flush=True)
+        assert np.array_equal(z, za) and np.array_equal(e, ea)
+        assert (r.sigma_E, r.sigma_z, r.E0) == (ra.sigma_E, ra.sigma_z, ra.E0)
+        b.close()
+
+
 def test_batch_of_uneven_turn_counts_matches_alone():
     """Bunches leave the batch at different turns; the others continue."""
     specs = _specs()[:3]
