@@ -246,7 +246,7 @@ Batch::~Batch() {
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     d_bd_.release(); d_tiles_.release();
     for (auto &c : d_conv_) c.release();
-    d_red_.release();
+    d_red_.release(); d_list_.release();
     d_tmp_tiles_.release(); d_tmp_list_.release(); d_ranges_.release();
 }
 
@@ -270,7 +270,7 @@ hm_status Batch::upload_descriptors() {
 
 // launch tables of the active bunches
 hm_status Batch::build_tables() {
-    std::vector<int32_t> tiles, conv[kConvKinds], red;
+    std::vector<int32_t> tiles, conv[kConvKinds], red, list;
     pass_lds_ = 0;
     spec_lds_ = 0;
     Fmax_ = 2;
@@ -294,6 +294,7 @@ hm_status Batch::build_tables() {
         if (!used_[i] || !q.active) continue;
         q.d.tile_begin = (int32_t)tiles.size();
         for (int t = 0; t < q.d.ntiles; ++t) tiles.push_back((int32_t)i);
+        list.push_back((int32_t)i);
         for (int r = 0; r < q.d.nred; ++r) {
             red.push_back((int32_t)i);
             red.push_back(r);
@@ -336,6 +337,7 @@ hm_status Batch::build_tables() {
     }
     ntiles_ = (int)tiles.size();
     nred_ = (int)red.size() / 2;
+    nlist_ = (int)list.size();
     // Workgroup ranges: contiguous runs of the tile table with equal numbers
     // of pairs (tiles of different bunches differ in size), for the pass and
     // for the interpolation sums (which only count wake + update_E0 bunches).
@@ -361,6 +363,9 @@ hm_status Batch::build_tables() {
                                s));
     HM_TRY(d_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
     HM_TRY(d_red_.ensure(std::max<size_t>(red.size(), 2)));
+    HM_TRY(d_list_.ensure(std::max<size_t>(list.size(), 1)));
+    if (!list.empty())
+        HIP_TRY(hipMemcpyAsync(d_list_.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
     if (!red.empty())
         HIP_TRY(hipMemcpyAsync(d_red_.p, red.data(), red.size() * 4, hipMemcpyHostToDevice, s));
     for (int k = 0; k < kConvKinds; ++k) {
@@ -657,7 +662,10 @@ hm_status Batch::one_turn() {
     HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, d_ranges_.p, pass_grid_, pass_lds_,
                              std_only_ ? 1 : 0, s));
     if (timing_) HIP_TRY(hipEventRecord(e1, s));
-    if (any_reduce_) HIP_TRY(launch_bin_reduce(d_bd_.p, d_red_.p, nred_, s));
+    const bool whole = nlist_ >= kWholeMinBunches;
+    if (any_reduce_)
+        HIP_TRY(whole ? launch_bin_reduce_whole(d_bd_.p, d_list_.p, nlist_, s)
+                      : launch_bin_reduce(d_bd_.p, d_red_.p, nred_, s));
     if (any_wake_) {
         HIP_TRY(launch_conv(CONV_FUSED, d_bd_.p, d_conv_[CONV_FUSED].p, nconv_[CONV_FUSED], Fmax_, s));
         HIP_TRY(launch_conv(CONV_FWD, d_bd_.p, d_conv_[CONV_FWD].p, nconv_[CONV_FWD], Fmax_, s));
@@ -670,7 +678,8 @@ hm_status Batch::one_turn() {
     if (any_wake_e0_)   // two workgroups per CU when both windows fit the LDS
         HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, d_ranges_.p + pass_grid_ + 1, interp_grid_,
                                   interp_lds_, interp_deep_, s));
-    HIP_TRY(launch_finalize(d_bd_.p, d_red_.p, nred_, s));
+    HIP_TRY(whole ? launch_finalize_whole(d_bd_.p, d_list_.p, nlist_, s)
+                  : launch_finalize(d_bd_.p, d_red_.p, nred_, s));
     return HM_OK;
 }
 

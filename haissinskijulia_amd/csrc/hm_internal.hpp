@@ -179,7 +179,12 @@ class Batch {
     bool interp_deep_ = false;   // one interpolation workgroup per CU: deeper rings
     DevBuf<int32_t> d_conv_[kConvKinds];   // convolution items of each ConvKind
     DevBuf<int32_t> d_red_;       // (bunch, slice) items of the reductions
-    int ntiles_ = 0, nred_ = 0;
+    DevBuf<int32_t> d_list_;      // the active bunches
+    int ntiles_ = 0, nred_ = 0, nlist_ = 0;
+    // one workgroup per bunch for the reductions once the bunches alone fill
+    // the device (no cross-workgroup counts, whose device-scope fences cost
+    // more than the slices gain there)
+    static constexpr int kWholeMinBunches = 16;
     int nconv_[kConvKinds] = {0, 0, 0, 0, 0};
     size_t spec_lds_ = 0;   // LDS of the spectral-sum workgroups
     size_t pass_lds_ = 0;

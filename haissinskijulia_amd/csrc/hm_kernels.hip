@@ -842,16 +842,9 @@ __device__ inline bool slices_done(const BunchDev &B, int r, const double (&v)[N
 // the binned count, and the interpolation cell of the centroid (where the LDS
 // windows of P are placed; the placement changes no result); the turn
 // monitor's centroid, sigma_z and binned count.  One workgroup per slice.
-__global__ __launch_bounds__(kRedThreads) void k_bin_reduce(const BunchDev *__restrict__ bds,
-                                                            const int32_t *__restrict__ items) {
-    __shared__ double s_red[4 * 16];
-    const BunchDev &B = bds[items[2 * blockIdx.x]];
-    const int r = items[2 * blockIdx.x + 1];
-    if (!(B.flags & (F_WAKE | F_E0))) return;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};   // sum z, sum z^2, sum (dE'-dE_init), binned count
-    slice_sum<4>(B.part, B.ntiles * kTileWaves, r, v, s_red);
-    double t[4];
-    if (!slices_done<4>(B, r, v, t) || threadIdx.x != 0) return;
+// The turn scalars from the reduced pass sums t = {sum z, sum z^2,
+// sum (dE'-dE_init), binned count}.
+__device__ inline void bin_scalars(const BunchDev &B, const double (&t)[4]) {
     TurnScalars &sc = *B.sc;
     sc.sum_dE = t[2];
     if (!(B.flags & F_WAKE)) return;
@@ -872,6 +865,106 @@ __global__ __launch_bounds__(kRedThreads) void k_bin_reduce(const BunchDev *__re
         m[1] = sz;
         m[3] = t[3];
     }
+}
+
+// The sum over the nred slices of a per-wave partial array by one 1024-thread
+// workgroup, with the arithmetic of slice_sum (256 threads) and slices_done:
+// the workgroup's four groups of four waves take four slices at a time, each
+// group exactly as a 256-thread slice workgroup, and thread 0 adds the slice
+// sums in slice order.  Same bits as the sliced form, no cross-workgroup
+// count (for batches of many bunches, where the bunches fill the device).
+template <int NV>
+__device__ inline void whole_sum(const BunchDev &B, const double *a, double (&t)[NV]) {
+    __shared__ double s_red[4][NV * 16];
+    __shared__ double s_slice[4][NV];
+    const int n = B.ntiles * kTileWaves, nred = B.nred;
+    const int g = (int)threadIdx.x >> 8, tl = (int)threadIdx.x & 255;
+    const int lane = tl & 63, wave = tl >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) t[j] = 0.0;
+    for (int r0 = 0; r0 < nred; r0 += 4) {
+        const int r = r0 + g;
+        double v[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] = 0.0;
+        if (r < nred) {   // slice_sum's loads and order, thread tl of slice r
+            const int e0 = r * kRedEntries + tl;
+            double x[kRedPer][NV];
+#pragma unroll
+            for (int k = 0; k < kRedPer; ++k) {
+                const int e = e0 + k * kRedThreads;
+                if constexpr (NV == 4) {
+                    double2 lo = make_double2(0.0, 0.0), hi = make_double2(0.0, 0.0);
+                    if (e < n) {
+                        const g_f64x2 *q = (const g_f64x2 *)a + 2 * (int64_t)e;
+                        lo = ld2(q);
+                        hi = ld2(q + 1);
+                    }
+                    x[k][0] = lo.x;
+                    x[k][1] = lo.y;
+                    x[k][2] = hi.x;
+                    x[k][3] = hi.y;
+                } else {
+                    x[k][0] = e < n ? ((const g_f64 *)a)[e] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kRedPer; ++k)
+#pragma unroll
+                for (int j = 0; j < NV; ++j) v[j] += x[k][j];
+        }
+        // block_sum's order within the group: wave butterflies, then its four waves
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v[j] += __shfl_xor(v[j], off, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int j = 0; j < NV; ++j) s_red[g][j * 16 + wave] = v[j];
+        __syncthreads();
+        if (tl == 0)
+#pragma unroll
+            for (int j = 0; j < NV; ++j) {
+                double x = 0.0;
+                for (int w = 0; w < 4; ++w) x += s_red[g][j * 16 + w];
+                s_slice[g][j] = x;
+            }
+        __syncthreads();
+        if (threadIdx.x == 0)   // slices_done: one slice as it is, else from zero in order
+            for (int q = 0; q < 4 && r0 + q < nred; ++q)
+#pragma unroll
+                for (int j = 0; j < NV; ++j) t[j] = nred == 1 ? s_slice[q][j] : t[j] + s_slice[q][j];
+        __syncthreads();
+    }
+}
+
+// Once per turn and bunch, after the pass: the pass's per-wave sums reduced
+// into the turn scalars.  sum (dE'-dE_init) for the E0 update; for a wake
+// bunch sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386),
+// the binned count, and the interpolation cell of the centroid (where the LDS
+// windows of P are placed; the placement changes no result); the turn
+// monitor's centroid, sigma_z and binned count.  One workgroup per slice.
+__global__ __launch_bounds__(kRedThreads) void k_bin_reduce(const BunchDev *__restrict__ bds,
+                                                            const int32_t *__restrict__ items) {
+    __shared__ double s_red[4 * 16];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int r = items[2 * blockIdx.x + 1];
+    if (!(B.flags & (F_WAKE | F_E0))) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};   // sum z, sum z^2, sum (dE'-dE_init), binned count
+    slice_sum<4>(B.part, B.ntiles * kTileWaves, r, v, s_red);
+    double t[4];
+    if (!slices_done<4>(B, r, v, t) || threadIdx.x != 0) return;
+    bin_scalars(B, t);
+}
+
+// k_bin_reduce with one 1024-thread workgroup per bunch (whole_sum)
+__global__ __launch_bounds__(1024) void k_bin_reduce_whole(const BunchDev *__restrict__ bds,
+                                                           const int32_t *__restrict__ list) {
+    const BunchDev &B = bds[list[blockIdx.x]];
+    if (!(B.flags & (F_WAKE | F_E0))) return;
+    double t[4];
+    whole_sum<4>(B, B.part, t);
+    if (threadIdx.x == 0) bin_scalars(B, t);
 }
 
 __device__ __forceinline__ double line_density(const BunchDev &B, const uint32_t *h, int i,
@@ -1432,21 +1525,15 @@ __global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restri
 
 // The E0 update (evolution.jl:419-429) from this turn's sums, and the scalars
 // of the next turn.  One workgroup per slice of the interpolation sums.
-__global__ __launch_bounds__(kRedThreads) void k_finalize(const BunchDev *__restrict__ bds,
-                                                          const int32_t *__restrict__ items) {
-    __shared__ double s_red[16];
-    const BunchDev &B = bds[items[2 * blockIdx.x]];
-    const int r = items[2 * blockIdx.x + 1];
+// The E0 update (evolution.jl:419-429) from this turn's sums (psum: the sum of
+// the interpolated potential over the bunch), and the scalars of the next turn.
+__device__ inline void finish_turn(const BunchDev &B, double psum) {
     const uint32_t fl = B.flags;
-    double v[1] = {0.0};   // sum of the interpolated P (the sum of dE'-dE_init is in sc)
-    if ((fl & F_E0) && (fl & F_WAKE)) slice_sum<1>(B.ipart, B.ntiles * kTileWaves, r, v, s_red);
-    double t[1];
-    if (!slices_done<1>(B, r, v, t) || threadIdx.x != 0) return;
     TurnScalars s = *B.sc;
     double E0 = s.E0;
     if (s.drift_skip) s.drift_skipped_count += 1;
     if (fl & F_E0) {                                  // evolution.jl:419-429
-        const double mean = (s.sum_dE - t[0]) / (double)B.n;
+        const double mean = (s.sum_dE - psum) / (double)B.n;
         E0 = E0 + mean;
         s.m_prev = mean;
         s.pending_m = 1;
@@ -1458,6 +1545,30 @@ __global__ __launch_bounds__(kRedThreads) void k_finalize(const BunchDev *__rest
     s.turn += 1;
     turn_scalars(B, E0, s);
     *B.sc = s;
+}
+
+// finish_turn after the sum of the interpolation sums, one workgroup per slice
+__global__ __launch_bounds__(kRedThreads) void k_finalize(const BunchDev *__restrict__ bds,
+                                                          const int32_t *__restrict__ items) {
+    __shared__ double s_red[16];
+    const BunchDev &B = bds[items[2 * blockIdx.x]];
+    const int r = items[2 * blockIdx.x + 1];
+    const uint32_t fl = B.flags;
+    double v[1] = {0.0};   // sum of the interpolated P (the sum of dE'-dE_init is in sc)
+    if ((fl & F_E0) && (fl & F_WAKE)) slice_sum<1>(B.ipart, B.ntiles * kTileWaves, r, v, s_red);
+    double t[1];
+    if (!slices_done<1>(B, r, v, t) || threadIdx.x != 0) return;
+    finish_turn(B, t[0]);
+}
+
+// k_finalize with one 1024-thread workgroup per bunch (whole_sum)
+__global__ __launch_bounds__(1024) void k_finalize_whole(const BunchDev *__restrict__ bds,
+                                                         const int32_t *__restrict__ list) {
+    const BunchDev &B = bds[list[blockIdx.x]];
+    const uint32_t fl = B.flags;
+    double t[1] = {0.0};
+    if ((fl & F_E0) && (fl & F_WAKE)) whole_sum<1>(B, B.ipart, t);
+    if (threadIdx.x == 0) finish_turn(B, t[0]);
 }
 
 __global__ void k_turn_scalars(const BunchDev *__restrict__ bd, double E0, TurnScalars *out) {
@@ -1822,6 +1933,18 @@ hipError_t launch_finalize(const BunchDev *bds, const int32_t *items, int nitems
 hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *items, int nitems, hipStream_t s) {
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bin_reduce, dim3(nitems), dim3(kRedThreads), 0, s, bds, items);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_reduce_whole(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_reduce_whole, dim3(nb), dim3(1024), 0, s, bds, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_whole(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize_whole, dim3(nb), dim3(1024), 0, s, bds, list);
     return hipGetLastError();
 }
 

@@ -84,6 +84,12 @@ hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, con
 // (bunch, slice) pairs, red_slices(ntiles) slices per bunch.
 hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *items, int nitems, hipStream_t s);
 hipError_t launch_finalize(const BunchDev *bds, const int32_t *items, int nitems, hipStream_t s);
+// the same two reductions with one 1024-thread workgroup per listed bunch and
+// no cross-workgroup count: the same bits (for batches of many bunches)
+hipError_t launch_bin_reduce_whole(const BunchDev *bds, const int32_t *bunch_list, int nb,
+                                   hipStream_t s);
+hipError_t launch_finalize_whole(const BunchDev *bds, const int32_t *bunch_list, int nb,
+                                 hipStream_t s);
 // FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
 hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,
                            hipStream_t s);

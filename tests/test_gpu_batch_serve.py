@@ -66,13 +66,16 @@ def test_batch_equals_each_bunch_alone_bitwise(turns):
         b.close()
 
 
-def test_spectral_sum_convolution_equals_block_convolution_bitwise():
-    # A bunch of 4096 bins alone re-reads ~4 MB of spectra per turn and takes
-    # the one-kernel inverse phase (k_conv_inv); eight of them in a batch take
-    # the spectral sums per frequency chunk (k_conv_spec, k_conv_out).  Both
-    # add the same products in the same order: the same bits.
-    specs = [dict(n=20_000 + 3_000 * i, nbins=4096, V=2.6e6 + 0.2e6 * i, charge=(1 + i) * 1e-9,
-                  seed=40 + i, flags=(True, True, True, True, True)) for i in range(8)]
+def test_many_bunch_forms_equal_alone_bitwise():
+    # Alone, a bunch of 4096 bins re-reads ~4 MB of spectra per turn and takes
+    # the one-kernel inverse phase (k_conv_inv), and its reductions run one
+    # workgroup per slice of tile sums.  Sixteen of them in a batch take the
+    # spectral sums per frequency chunk (k_conv_spec, k_conv_out) and one
+    # workgroup per bunch for the reductions (k_bin_reduce_whole,
+    # k_finalize_whole; up to four slices here).  Each form adds the same
+    # numbers in the same order: the same bits.
+    specs = [dict(n=20_000 + 25_000 * i, nbins=4096, V=2.6e6 + 0.1e6 * i, charge=(1 + i % 8) * 1e-9,
+                  seed=40 + i, flags=(True, True, True, True, True)) for i in range(16)]
     alone = []
     for s in specs:
         b = _bunch(s)
