@@ -273,6 +273,29 @@ __device__ __forceinline__ void wave_total(double (&v)[NV]) {
     }
 }
 
+// The wave totals of four values, in a fixed order, the total of v[q] in the
+// lanes with (lane & 3) == q (lane 60 + q among them).  The quantities are
+// transposed across lanes while they are summed: lanes l and l^1 exchange two
+// values each and keep the sums of the other two, lanes l and l^2 one, so
+// after two steps every lane holds one quantity summed over its quad; four
+// more steps (row rotations by 4 and 8, then the rows by lane xor 16 and 32)
+// sum it over the wave.  A third of the instructions of four wave_total<1>.
+__device__ __forceinline__ double wave_total4(const double (&v)[4]) {
+    const int l = (int)threadIdx.x & 63;
+    const bool odd = (l & 1) != 0, hi = (l & 2) != 0;
+    // quad_perm [1,0,3,2]: quantities 0, 2 (even lanes) or 1, 3 (odd lanes)
+    const double ka = (odd ? v[1] : v[0]) + dpp_move<0xB1, 0xF>(odd ? v[0] : v[1]);
+    const double kb = (odd ? v[3] : v[2]) + dpp_move<0xB1, 0xF>(odd ? v[2] : v[3]);
+    double k = hi ? kb : ka;
+    const double s = hi ? ka : kb;
+    k = k + dpp_move<0x4E, 0xF>(s);      // quad_perm [2,3,0,1]: quantity l & 3
+    k = k + dpp_move<0x124, 0xF>(k);     // row_ror:4
+    k = k + dpp_move<0x128, 0xF>(k);     // row_ror:8
+    k = k + __shfl_xor(k, 16, 64);
+    k = k + __shfl_xor(k, 32, 64);
+    return k;
+}
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -642,11 +665,10 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
         if (!wake) { v[0] = 0.0; v[1] = 0.0; }
         // per-wave partial sums: no workgroup barrier per tile, so a wave never
         // waits for the others of its workgroup between tiles
-        wave_total<4>(v);
-        if ((threadIdx.x & 63) == 63) {
-            double *o = B.part + 4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6));
-            o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
-        }
+        const double tot = wave_total4(v);
+        const int lane = (int)threadIdx.x & 63;
+        if (lane >= 60)
+            B.part[4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6)) + (lane - 60)] = tot;
     }
 }
 
