@@ -506,7 +506,7 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
 //
 // One call covers the run of tiles [lt0, lt1) of bunch B that this workgroup
 // owns.  Tiles are whole multiples of the block size, so thread t's pairs in
-// tile lt are lt*ppt + t + k*blockDim.x in ascending k, exactly as for a
+// tile lt are lt*ppt + t + k*kPassThreads in ascending k, exactly as for a
 // tile taken alone: the loop keeps its one-pair prefetch running across the
 // tile boundaries and writes each tile's per-wave sums at the tile's end.
 // KM, how the kick reads P: kFullWindow (the LDS window holds the whole table:
@@ -550,12 +550,18 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
         // the waits there count only the loads issued in the loop
         __builtin_amdgcn_s_waitcnt(0x0F70);
     }
+    // The pair stride is the block size (kPassThreads).  The gather-first loop
+    // takes it as the constant (its register allocation then needs no reload
+    // of the block size per tile: 1e8 pass -2%); the other loops read
+    // blockDim.x, with which their allocation spills less (1e7 pass -2%,
+    // 1e5 -4% against the constant).
+    const int64_t step = KM == kGatherFirst ? (int64_t)kPassThreads : (int64_t)blockDim.x;
     for (int lt = lt0; lt < lt1; ++lt) {
         const int64_t p0 = (int64_t)lt * ppt;
         const int64_t p1 = min(p0 + ppt, B.npairs);
         const int64_t pf = min(p1, full);
         double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
-        for (int64_t p = p0 + threadIdx.x; p < pf; p += blockDim.x) {
+        for (int64_t p = p0 + threadIdx.x; p < pf; p += step) {
             const double2 zz = zn, ee = en;
             // The previous turn's potential at (P[k], P[k+1]): read from the LDS
             // window (clamped index, always valid), replaced by one 16-byte
@@ -567,7 +573,7 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             // loads changes no result.
             constexpr bool gather = KM != kFullWindow;
             constexpr bool gather_first = KM == kGatherFirst;
-            const int64_t pn = p + blockDim.x;
+            const int64_t pn = p + step;
             // the next pair (the next tile's first at a tile end); past the end
             // of the run the current pair is read again and not used, so the
             // load is unconditional and zn, en need no copy of their old value
@@ -1225,10 +1231,10 @@ __device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0,
     const int wlo = window_lo(B.sc->centre_bin, w, nb);
     const unsigned wpairs = (unsigned)(w - 1);
     __syncthreads();   // the previous run's window is no longer read
-    for (int k = threadIdx.x; k < w; k += blockDim.x) sP[k] = P[wlo + k];
+    for (int k = threadIdx.x; k < w; k += kInterpThreads) sP[k] = P[wlo + k];
     __syncthreads();
     const int64_t n = B.n, ppt = B.pairs_per_tile, full = n >> 1;
-    const int64_t bs = blockDim.x;
+    const int64_t bs = kInterpThreads;
     const int K = (int)(ppt / bs);   // pairs of each thread in a full tile
     const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
     const g_f64 *zp = (const g_f64 *)B.z;
@@ -1329,10 +1335,10 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
     const int wlo = window_lo(B.sc->centre_bin, w, nb);
     const unsigned wpairs = (unsigned)(w - 1);
     __syncthreads();   // the previous run's window is no longer read
-    for (int k = threadIdx.x; k < w; k += blockDim.x) sP[k] = P[wlo + k];
+    for (int k = threadIdx.x; k < w; k += kInterpThreads) sP[k] = P[wlo + k];
     __syncthreads();
     const int64_t n = B.n, ppt = B.pairs_per_tile, full = n >> 1;
-    const int tid = (int)threadIdx.x, ibs = (int)blockDim.x;
+    const int tid = (int)threadIdx.x, ibs = kInterpThreads;
     const int K = (int)(ppt / ibs);   // pairs of each thread in a full tile
     const double c0 = B.c_first, c1 = B.c_last, inv = B.inv_s;
     const g_f64 *zp = (const g_f64 *)B.z;
@@ -1422,9 +1428,9 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
 // large tiles, the run-wide ring no pipeline start per tile for small ones.
 template <int G>
 __device__ __forceinline__ void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
-    if (G > 4 && B.pairs_per_tile >= G * (int64_t)blockDim.x)
+    if (G > 4 && B.pairs_per_tile >= G * (int64_t)kInterpThreads)
         interp_sum_run_tiles<G>(B, lt0, lt1);
-    else if (B.pairs_per_tile >= 4 * (int64_t)blockDim.x)
+    else if (B.pairs_per_tile >= 4 * (int64_t)kInterpThreads)
         interp_sum_run_tiles<4>(B, lt0, lt1);
     else
         interp_sum_run_flat(B, lt0, lt1);
