@@ -646,7 +646,7 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             v[2] += d1;
             // The prefetched pair is complete before this pair's stores are
             // issued: a wait for a load behind a store would wait for the store.
-            if (KM == kFullWindow) __builtin_amdgcn_s_waitcnt(0x0F70);
+            __builtin_amdgcn_s_waitcnt(0x0F70);
             // streaming stores: the bunch is far larger than the caches, which
             // are left to the bin phase's tables
             stnt((g_f64x2 *)(zp + 2 * p), z0, z1);
