@@ -567,10 +567,9 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             // window (clamped index, always valid), replaced by one 16-byte
             // global gather for the particles outside a partial window.  With
             // gather_first the gathers are issued before the next pair's
-            // prefetch and consumed after the RF and QE terms, so their wait
-            // leaves the prefetch in flight and that arithmetic covers their
-            // latency.  Otherwise the prefetch goes first.  The order of the
-            // loads changes no result.
+            // prefetch and consumed after the RF and QE terms, whose
+            // arithmetic covers their latency.  Otherwise the prefetch goes
+            // first.  The order of the loads changes no result.
             constexpr bool gather = KM != kFullWindow;
             constexpr bool gather_first = KM == kGatherFirst;
             const int64_t pn = p + step;
