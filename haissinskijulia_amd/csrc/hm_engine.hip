@@ -584,7 +584,9 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         d.logF = ilog2(d.F);
         d.nA = (int32_t)((nb + Bk - 1) / Bk);
         d.nJ = (int32_t)((nw + Bk - 1) / Bk);
-        if ((size_t)nb * 4 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
+        // the histogram in LDS only with room for a window of at least two
+        // entries of P beside it (the all-flags pass relies on that)
+        if ((size_t)nb * 4 + 16 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
         {   // the pass's LDS window of P: what the histogram leaves of the LDS
             const int64_t cap = ((int64_t)ctx_->lds_max - 1024 - (int64_t)pass_hist_bytes(d)) / 8;
             d.win = cap >= 2 ? (int32_t)std::min<int64_t>(nb, cap) : 0;

@@ -590,7 +590,10 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
                 interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
                 interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
                 const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
-                if (KM == kFullWindow || wpairs) {   // a full window has wpairs >= 1
+                // a full window, and any window of the all-flags pass (its
+                // histogram is in LDS, so its window has at least two entries),
+                // has wpairs >= 1
+                if (KM == kFullWindow || FX != 0 || wpairs) {
                     const unsigned i0 = min(j0, wpairs - 1), i1 = min(j1, wpairs - 1);
                     q0 = make_double2(sP[i0], sP[i0 + 1]);
                     q1 = make_double2(sP[i1], sP[i1 + 1]);
