@@ -1224,7 +1224,7 @@ __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__res
 // streaming loads (the next tile's first group at a tile's end) issued before
 // this group's sums; the group loop has the same trip count in every lane.
 // A tile's sums therefore depend on the tile and the block size only (not on G).
-template <int G>
+template <int G, bool FULLW>
 __device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1) {
     extern __shared__ double sP[];
     const int nb = B.nbins;
@@ -1302,8 +1302,8 @@ __device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0,
                     double t0 = sP[i0] * (1.0 - w0) + sP[i0 + 1] * w0;
                     double t1 = sP[i1] * (1.0 - w1) + sP[i1 + 1] * w1;
                     // outside the window: the pair table, with the wait in the branch
-                    if (j0 >= wpairs) t0 = gather_interp(P2 + k0, w0);
-                    if (j1 >= wpairs) t1 = gather_interp(P2 + k1, w1);
+                    if (!FULLW && j0 >= wpairs) t0 = gather_interp(P2 + k0, w0);
+                    if (!FULLW && j1 >= wpairs) t1 = gather_interp(P2 + k1, w1);
                     v[0] += t0;
                     v[0] += t1;
                 }
@@ -1329,6 +1329,7 @@ __device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0,
 // streaming loads in flight, so a tile of one pair per thread costs no
 // pipeline start.  A tile's sum is written when its last pair is added; it
 // depends on the tile and the block size only.
+template <bool FULLW>
 __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) {
     extern __shared__ double sP[];
     const int nb = B.nbins;
@@ -1408,8 +1409,8 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
                 double t0 = sP[i0] * (1.0 - w0) + sP[i0 + 1] * w0;
                 double t1 = sP[i1] * (1.0 - w1) + sP[i1 + 1] * w1;
                 // outside the window: the pair table, with the wait in the branch
-                if (j0 >= wpairs) t0 = gather_interp(P2 + k0, w0);
-                if (j1 >= wpairs) t1 = gather_interp(P2 + k1, w1);
+                if (!FULLW && j0 >= wpairs) t0 = gather_interp(P2 + k0, w0);
+                if (!FULLW && j1 >= wpairs) t1 = gather_interp(P2 + k1, w1);
                 v += t0;
                 v += t1;
             }
@@ -1428,14 +1429,24 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
 // LDS permutes, the small-tile form by DPP moves).  Which form a bunch takes
 // depends on its size alone.  The per-tile groups cost less bookkeeping for
 // large tiles, the run-wide ring no pipeline start per tile for small ones.
+// FULLW: the window holds the whole table (no gathers), so the compiler's
+// waits for the ring's slots count only the ring's own loads.
+template <int G, bool FULLW>
+__device__ __forceinline__ void interp_sum_run_w(const BunchDev &B, int lt0, int lt1) {
+    if (G > 4 && B.pairs_per_tile >= G * (int64_t)kInterpThreads)
+        interp_sum_run_tiles<G, FULLW>(B, lt0, lt1);
+    else if (B.pairs_per_tile >= 4 * (int64_t)kInterpThreads)
+        interp_sum_run_tiles<4, FULLW>(B, lt0, lt1);
+    else
+        interp_sum_run_flat<FULLW>(B, lt0, lt1);
+}
+
 template <int G>
 __device__ __forceinline__ void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
-    if (G > 4 && B.pairs_per_tile >= G * (int64_t)kInterpThreads)
-        interp_sum_run_tiles<G>(B, lt0, lt1);
-    else if (B.pairs_per_tile >= 4 * (int64_t)kInterpThreads)
-        interp_sum_run_tiles<4>(B, lt0, lt1);
+    if (B.iwin >= B.nbins)
+        interp_sum_run_w<G, true>(B, lt0, lt1);
     else
-        interp_sum_run_flat(B, lt0, lt1);
+        interp_sum_run_w<G, false>(B, lt0, lt1);
 }
 
 // The interpolation sums of the wake + update_E0 bunches over a tile table:
