@@ -556,11 +556,25 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
     // blockDim.x, with which their allocation spills less (1e7 pass -2%,
     // 1e5 -4% against the constant).
     const int64_t step = KM == kGatherFirst ? (int64_t)kPassThreads : (int64_t)blockDim.x;
+    // per-wave partial sums of a tile: no workgroup barrier per tile, so a wave
+    // never waits for the others of its workgroup between tiles
+    auto flush = [&](const double (&v)[4], int lt) {
+        const double tot = wave_total4(v);
+        const int lane = (int)threadIdx.x & 63;
+        if (lane >= 60)
+            B.part[4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6)) + (lane - 60)] = tot;
+    };
+    // the single particle of an odd bunch sits in the bunch's last tile, the
+    // last of this run if the run has it: that tile is flushed after the loop
+    const bool odd_tile = (n & 1) && full >= (int64_t)(lt1 - 1) * ppt &&
+                          full < min((int64_t)lt1 * ppt, B.npairs);
+    double v[4];   // sum z, sum z^2, sum (dE'-dE_init), in-range count
     for (int lt = lt0; lt < lt1; ++lt) {
         const int64_t p0 = (int64_t)lt * ppt;
         const int64_t p1 = min(p0 + ppt, B.npairs);
         const int64_t pf = min(p1, full);
-        double v[4] = {0.0, 0.0, 0.0, 0.0};  // sum z, sum z^2, sum (dE'-dE_init), in-range count
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = 0.0;
         for (int64_t p = p0 + threadIdx.x; p < pf; p += step) {
             const double2 zz = zn, ee = en;
             // The previous turn's potential at (P[k], P[k+1]): read from the LDS
@@ -654,8 +668,14 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             stnt((g_f64x2 *)(zp + 2 * p), z0, z1);
             stnt((g_f64x2 *)(ep + 2 * p), e0, e1);
         }
-        // the single particle of an odd bunch (last pair)
-        if ((n & 1) && threadIdx.x == 0 && full >= p0 && full < p1) {
+        if (odd_tile && lt == lt1 - 1) break;   // v kept for the single particle
+        if (!wake) { v[0] = 0.0; v[1] = 0.0; }
+        flush(v, lt);
+    }
+    // the single particle of an odd bunch (last pair), out of the tile loop so
+    // that its inlined turn does not add to the loop's register pressure
+    if (odd_tile && lt0 < lt1) {
+        if (threadIdx.x == 0) {
             const int64_t i = n - 1;
             double z0 = zp[i], e0 = ep[i], r0 = 0.0, r1 = 0.0;
             if (qe) {
@@ -671,12 +691,7 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             ep[i] = e0;
         }
         if (!wake) { v[0] = 0.0; v[1] = 0.0; }
-        // per-wave partial sums: no workgroup barrier per tile, so a wave never
-        // waits for the others of its workgroup between tiles
-        const double tot = wave_total4(v);
-        const int lane = (int)threadIdx.x & 63;
-        if (lane >= 60)
-            B.part[4 * ((int64_t)lt * kTileWaves + (threadIdx.x >> 6)) + (lane - 60)] = tot;
+        flush(v, lt1 - 1);
     }
 }
 
