@@ -550,12 +550,10 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
         // the waits there count only the loads issued in the loop
         __builtin_amdgcn_s_waitcnt(0x0F70);
     }
-    // The pair stride is the block size (kPassThreads).  The gather-first loop
-    // takes it as the constant (its register allocation then needs no reload
-    // of the block size per tile: 1e8 pass -2%); the other loops read
-    // blockDim.x, with which their allocation spills less (1e7 pass -2%,
-    // 1e5 -4% against the constant).
-    const int64_t step = KM == kGatherFirst ? (int64_t)kPassThreads : (int64_t)blockDim.x;
+    // The pair stride is the block size, as the compile-time constant: read
+    // from the dispatch packet, blockDim.x was reloaded per tile with a vector
+    // load and a full wait (1e8 pass +2%).
+    const int64_t step = kPassThreads;
     // per-wave partial sums of a tile: no workgroup barrier per tile, so a wave
     // never waits for the others of its workgroup between tiles
     auto flush = [&](const double (&v)[4], int lt) {
