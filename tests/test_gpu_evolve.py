@@ -78,6 +78,32 @@ def test_device_philox_stream_matches_oracle(dev):
     _check(*_run_both(p, 8192, 512, normals=False, turn_offset=12345))
 
 
+def test_turn_monitor_matches_oracle(dev):
+    # the turn-by-turn record (hm_options.monitor): centroid and sigma_z after
+    # the drift, E0 after the turn, binned count; injected normals, 40 turns
+    turns, n, nb = 40, 20_000, 512
+    p = W.ring_3gev_lattice(n_turns=turns)
+    res = [W.RESONATOR_BB]
+    z0, e0 = O.generate_gaussian(3, 0.0, 4e-3, 0.0, 3e6, n)
+    nrm = np.vstack([O.normals(103, t, 0, n) for t in range(turns)])
+    mon_o = np.zeros((turns, 4))
+    O.evolve(z0, e0, nb, p, normals=nrm, seed=103, resonators=res, charge=5e-9, monitor=mon_o)
+    b = H.DeviceBunch(n, dev)
+    b.upload(z0, e0)
+    r = b.evolve(H.SimulationParameters.from_dict(p), nb,
+                 H.TrackingOptions(seed=103, normals=nrm, resonators=res, charge=5e-9,
+                                   monitor=True))
+    b.close()
+    mon = r.monitor
+    print("case monitor", turns, n, nb, flush=True)
+    assert mon.shape == (turns, 4)
+    assert np.array_equal(mon[:, 3], mon_o[:, 3])          # binned counts: exact
+    assert np.allclose(mon[:, 2], mon_o[:, 2], rtol=1e-14, atol=0.0)   # E0
+    sz = np.abs(mon_o[:, 1])
+    assert np.all(np.abs(mon[:, 0] - mon_o[:, 0]) <= 1e-12 * sz)      # centroid, per sigma_z
+    assert np.allclose(mon[:, 1], mon_o[:, 1], rtol=1e-12, atol=0.0)  # sigma_z
+
+
 def test_large_bunch_matches_oracle(dev):
     # 1.7e7 particles: tiles of 9216 pairs (nine per thread), so the
     # interpolation sums run their eight-deep ring with a partial second group,
