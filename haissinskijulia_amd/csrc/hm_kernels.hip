@@ -879,12 +879,6 @@ __device__ inline bool slices_done(const BunchDev &B, int r, const double (&v)[N
     return true;
 }
 
-// Once per turn and bunch, after the pass: the pass's per-wave sums reduced
-// into the turn scalars.  sum (dE'-dE_init) for the E0 update; for a wake
-// bunch sigma_z (evolution.jl:357), the wake current (evolution.jl:373-386),
-// the binned count, and the interpolation cell of the centroid (where the LDS
-// windows of P are placed; the placement changes no result); the turn
-// monitor's centroid, sigma_z and binned count.  One workgroup per slice.
 // The turn scalars from the reduced pass sums t = {sum z, sum z^2,
 // sum (dE'-dE_init), binned count}.
 __device__ inline void bin_scalars(const BunchDev &B, const double (&t)[4]) {
