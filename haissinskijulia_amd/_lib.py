@@ -13,9 +13,7 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# HAISSINSKI_MI355X_LIB selects another build of the same library (e.g. a
-# variant compiled for a measurement); the default is the in-tree build.
-LIB_PATH = os.environ.get("HAISSINSKI_MI355X_LIB") or os.path.join(HERE, "libhaissinski_mi355x.so")
+LIB_PATH = os.path.join(HERE, "libhaissinski_mi355x.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "haissinski_mi355x.h")
 
 _dp = C.POINTER(C.c_double)
@@ -49,7 +47,7 @@ class HmOptions(C.Structure):
         ("wake_table", _dp), ("delta_sigma", C.c_double), ("bin_half_width_sigmas", C.c_double),
         ("charge", C.c_double), ("n_resonators", C.c_int32), ("timing", C.c_int32),
         ("resonators", _dp), ("fft_block", C.c_int32), ("block_size", C.c_int32),
-        ("monitor", _dp),
+        ("monitor", _dp), ("serial", C.c_int32), ("fom_stats", C.c_int32),
     ]
 
 
@@ -68,7 +66,7 @@ class HmJob(C.Structure):
     _fields_ = [
         ("params", HmParams), ("opts", HmOptions), ("n", C.c_int64), ("nbins", C.c_int64),
         ("particle_seed", C.c_uint64), ("sigma_z", C.c_double), ("sigma_E", C.c_double),
-        ("fail_after_turns", C.c_int32), ("pad_", C.c_int32),
+        ("fail_after_turns", C.c_int32), ("fail_setup", C.c_int32),
     ]
 
 

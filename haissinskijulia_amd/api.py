@@ -14,8 +14,9 @@ Names, argument order and return values follow the reference
   ``longitudinal_evolve!`` of src/evolution.jl:206-212; mutates the particles
   and returns ``(σ_E, σ_z, E0)``.  A ``MI355X`` token in the ``comm`` slot
   selects the GPU (one GPU holds the whole bunch, so the MPI collectives of
-  the reference become on-device reductions; ``use_mpi`` is accepted and
-  ignored).
+  the reference become on-device reductions).  ``use_mpi = False`` selects
+  the reference's serial branch: two-pass ``compute_std`` spreads, no E0
+  update of an empty bunch, and the reference's error with the wake on.
 * stage functions ``rf_kick``, ``quantum_excitation``,
   ``synchrotron_radiation``, ``apply_wakefield_inplace`` and the helpers
   ``z_to_ϕ``, ``ϕ_to_z``, ``calc_rf_factor``, ``next_power_of_two``.
@@ -158,6 +159,8 @@ class TrackingOptions:
     fft_block: int = 0
     block_size: int = 0
     monitor: bool = False                # record per-turn centroid, sigma_z, E0, binned count
+    serial: bool = False                 # the use_mpi = false branch (longitudinal_evolve sets it)
+    fom_stats: bool = False              # final two-pass means and cor(z, ΔE) (figures of merit)
 
     def _c(self, keep: list, n: int, n_turns: int, nbins: int) -> HmOptions:
         o = HmOptions()
@@ -192,6 +195,8 @@ class TrackingOptions:
             keep.append(mon)
             o.monitor = dptr(mon)
             o._monitor = mon
+        o.serial = int(bool(self.serial))
+        o.fom_stats = int(bool(self.fom_stats))
         return o
 
 
@@ -211,7 +216,8 @@ class EvolveResult:
     bin_half_width: float
     mean_z: float = 0.0
     mean_dE: float = 0.0
-    corr_z_dE: float = 0.0    # Statistics.cor of the final bunch (two-pass, on the GPU)
+    corr_z_dE: float = 0.0    # Statistics.cor of the final bunch (two-pass, on the GPU;
+                              # NaN unless TrackingOptions.fom_stats)
     # per turn (TrackingOptions.monitor): z centroid and sigma_z after the drift
     # (wake on), E0 after the turn, binned count
     monitor: Optional[np.ndarray] = None
@@ -384,6 +390,7 @@ class Job:
     sigma_E: float
     options: TrackingOptions = field(default_factory=TrackingOptions)
     fail_after_turns: int = 0   # >0: stop the job early with HM_ERR_CANCELLED (failure hook)
+    fail_setup: bool = False    # its set-up inside the batch fails (admission error-path hook)
 
 
 @dataclass
@@ -429,6 +436,7 @@ class Server:
         j.sigma_z = float(job.sigma_z)
         j.sigma_E = float(job.sigma_E)
         j.fail_after_turns = int(job.fail_after_turns)
+        j.fail_setup = int(bool(job.fail_setup))
         jid = C.c_int64()
         check(_lib.load().hm_server_submit(self._h, C.byref(j), C.byref(jid)))
         return jid.value
@@ -438,9 +446,10 @@ class Server:
         common random numbers (same particle and QE seeds).  Returns their ids;
         ``fd_sensitivity`` turns the two results into dFoM/dV."""
         import dataclasses
-        lo = dataclasses.replace(job, params=job.params.replace(
+        opts = dataclasses.replace(job.options, fom_stats=True)   # the FoMs need cor(z, ΔE)
+        lo = dataclasses.replace(job, options=opts, params=job.params.replace(
             voltage=job.params.voltage * (1.0 - rel_step)))
-        hi = dataclasses.replace(job, params=job.params.replace(
+        hi = dataclasses.replace(job, options=opts, params=job.params.replace(
             voltage=job.params.voltage * (1.0 + rel_step)))
         return self.submit(lo), self.submit(hi)
 
@@ -492,8 +501,9 @@ def longitudinal_evolve(particles: Particles, params: SimulationParameters,
     carries the seed and the extensions; ``options.timing`` adds HIP-event
     timings, available afterwards as ``longitudinal_evolve.last``.
     """
+    import dataclasses
     dev = _dev(comm)
-    options = options or TrackingOptions()
+    options = dataclasses.replace(options or TrackingOptions(), serial=not use_mpi)
     z = particles.coordinates.z
     dE = particles.coordinates.ΔE
     n = z.size
@@ -680,6 +690,9 @@ def compute_fom(fom: FigureOfMerit, particles: Particles, results, comm=None) ->
     if isinstance(fom, EmittanceFoM):
         if isinstance(results, EvolveResult) and particles is None:
             cor = results.corr_z_dE
+            if math.isnan(cor):
+                raise ValueError("the result carries no cor(z, ΔE): evolve with "
+                                 "TrackingOptions(fom_stats=True) or pass the particles")
         else:
             cor = pair_stats(particles.coordinates.z, particles.coordinates.ΔE, comm)["cor"]
         return σ_z * σ_E * math.sqrt(1 - cor ** 2)

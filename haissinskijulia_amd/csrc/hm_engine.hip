@@ -200,6 +200,13 @@ hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t n
                 return fail(HM_ERR_ARG, "fft_block must be a power of two <= %d", kMaxFFT / 2);
         }
     }
+    if (o && o->serial && p->use_wakefield)   // evolution.jl:267 with comm = nothing
+        return fail(HM_ERR_ARG,
+                    "MethodError: no method matching initialize_mpi_buffers!(::SimulationBuffers{"
+                    "Float64}, ::Int64, ::Nothing, ::Int64) (the wakefield needs use_mpi = true)");
+    if (o && o->charge > 0 && o->delta_sigma > 0)
+        return fail(HM_ERR_ARG, "charge sets the current in coulombs: the line density is "
+                                "counts/n_h without the delta kernel (delta_sigma must be <= 0)");
     if (o && o->n_resonators > 0 && !o->resonators) return fail(HM_ERR_ARG, "resonators is NULL");
     if (o && o->block_size && o->block_size != kPassThreads)
         return fail(HM_ERR_ARG, "block_size must be 0 or %d", kPassThreads);
@@ -413,11 +420,14 @@ hm_status Batch::epilogue(const std::vector<int32_t> &which) {
     hipStream_t s = ctx_->stream;
     for (int32_t i : which) {
         RunBunch &q = rb_[(size_t)i];
-        // two-pass means, spreads and cor(z, dE) of the final bunch -> out[9..13]
-        // (the emittance figure of merit, figures_of_merit.jl:36-47)
-        HM_TRY(q.b->w.stats.ensure((size_t)3 * kStatBlocks));
-        HIP_TRY(hipMemsetAsync(q.d.out + 9, 0, 5 * sizeof(double), s));
-        HIP_TRY(launch_pair_stats(q.b->z, q.b->dE, q.b->n, q.d.out + 9, q.b->w.stats.p, s));
+        // two-pass means, spreads and cor(z, dE) of the final bunch -> out[9..13]:
+        // the emittance figure of merit (figures_of_merit.jl:36-47) and the
+        // compute_std spreads of the serial branch (evolution.jl:464-467)
+        if (q.o.fom_stats || q.o.serial) {
+            HM_TRY(q.b->w.stats.ensure((size_t)3 * kStatBlocks));
+            HIP_TRY(hipMemsetAsync(q.d.out + 9, 0, 5 * sizeof(double), s));
+            HIP_TRY(launch_pair_stats(q.b->z, q.b->dE, q.b->n, q.d.out + 9, q.b->w.stats.p, s));
+        }
         q.active = false;
         q.finished = true;
     }
@@ -426,8 +436,30 @@ hm_status Batch::epilogue(const std::vector<int32_t> &which) {
 }
 
 hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) {
-    hipStream_t s = ctx_->stream;
     std::vector<int32_t> fresh;
+    const hm_status st = add_slots(specs, fresh);
+    if (st != HM_OK) {
+        // no copy of this call may still read the caller's memory, and every
+        // slot taken here is released: nothing may launch on the bunches the
+        // caller frees after this failure
+        (void)hipStreamSynchronize(ctx_->stream);
+        for (int32_t i : fresh) {
+            used_[(size_t)i] = false;
+            rb_[(size_t)i].active = false;
+            rb_[(size_t)i].finished = false;
+            rb_[(size_t)i].b = nullptr;
+        }
+        dirty_ = true;
+        drop_graph();
+        return st;
+    }
+    if (idx)
+        for (int32_t i : fresh) idx->push_back(i);
+    return HM_OK;
+}
+
+hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32_t> &fresh) {
+    hipStream_t s = ctx_->stream;
     for (const RunBunch &spec : specs) {
         HM_TRY(validate(&spec.p, &spec.o, spec.b->n, spec.nbins));
         int slot = -1;
@@ -441,6 +473,8 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         RunBunch &q = rb_[(size_t)slot];
         q = spec;
         used_[(size_t)slot] = true;
+        fresh.push_back((int32_t)slot);
+        if (q.fail_setup) return fail(HM_ERR_CANCELLED, "set-up failed by request (test hook)");
         q.turns_done = 0;
         q.n_turns = q.p.n_turns;
         q.finished = false;
@@ -463,6 +497,7 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         if (p->use_excitation) fl |= F_QE;
         if (q.o.normals && p->use_excitation) fl |= F_INJ;
         if (q.o.charge > 0) fl |= F_CHARGE;
+        if (p->use_wakefield && (q.o.wake_table || q.o.n_resonators > 0)) fl |= F_LAG;
         d.flags = fl;
         d.mass = p->mass;
         d.inv_mass = 1.0 / p->mass;
@@ -512,8 +547,7 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         q.o.normals = nullptr;  // host memory of the caller: not kept
         q.o.wake_table = nullptr;
         q.o.resonators = nullptr;
-        fresh.push_back((int32_t)slot);
-        if (idx) idx->push_back(slot);
+        q.o.monitor = nullptr;
         // keep the wake description: the bin grid needs sigma_z0 first
         q.wake_copy.clear();
         q.res_copy.clear();
@@ -525,8 +559,18 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         }
     }
     HIP_TRY(hipStreamSynchronize(s));  // normals copies read the caller's memory
-    // initial spreads (evolution.jl:245-253)
+    // initial spreads (evolution.jl:245-253): compute_global_std, or for the
+    // serial branch compute_std (two passes) into out[0..1]
     HM_TRY(run_subset_moments(fresh, 0));
+    for (int32_t i : fresh) {
+        RunBunch &q = rb_[(size_t)i];
+        if (!q.o.serial || q.b->n < 2) continue;   // n < 2: both forms give 0
+        HM_TRY(q.b->w.stats.ensure((size_t)3 * kStatBlocks));
+        HIP_TRY(launch_pair_stats(q.b->z, q.b->dE, q.b->n, q.d.out + 9, q.b->w.stats.p, s));
+        HIP_TRY(hipMemcpyAsync(q.d.out, q.d.out + 12, sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(q.d.out + 1, q.d.out + 11, sizeof(double), hipMemcpyDeviceToDevice,
+                               s));
+    }
     std::vector<std::vector<double>> keep;   // host tables alive until the copies finish
     for (int32_t i : fresh) {
         RunBunch &q = rb_[(size_t)i];
@@ -554,7 +598,9 @@ hm_status Batch::add(const std::vector<RunBunch> &specs, std::vector<int> *idx) 
         const double wf = ((120.0 * kPi) * kSpeedLight) / (kPi * q.p.pipe_radius);   // evolution.jl:279
         const double ws = std::sqrt(jl_max_h(0.0, (2.0 * 30.0) / q.p.pipe_radius));  // evolution.jl:280
         const double inv_ctau = 1.0 / 4e-3;                                          // wakefield.jl:158
-        const double dsig = q.o.delta_sigma == 0.0 ? 0.15 : q.o.delta_sigma;
+        // with a charge the line density is counts/n_h: the delta kernel
+        // belongs to the reference's dimensionless current heuristic
+        const double dsig = q.o.charge > 0 ? -1.0 : (q.o.delta_sigma == 0.0 ? 0.15 : q.o.delta_sigma);
         for (int64_t k = 0; k < nb; ++k) {
             double wv;
             if (!q.wake_copy.empty()) {
@@ -795,9 +841,14 @@ hm_status Batch::result(int i, hm_result *r, double *cz, double *cE) {
     r->ms_conv_avg = ms_conv();
     r->pass_launches = pass_count_;
     r->bin_half_width = (q.d.flags & F_WAKE) ? q.d.e_last : 0.0;
-    r->mean_z = o[9];
-    r->mean_dE = o[10];
-    r->corr_z_dE = o[13];
+    const bool two_pass = (q.o.fom_stats || q.o.serial) && q.b->n >= 2;
+    if (q.o.serial && two_pass) {   // compute_std of the final bunch (evolution.jl:464-467)
+        r->sigma_E = o[12];
+        r->sigma_z = o[11];
+    }
+    r->mean_z = two_pass ? o[9] : o[7];
+    r->mean_dE = two_pass ? o[10] : o[8];
+    r->corr_z_dE = (q.o.fom_stats && q.b->n >= 2) ? o[13] : NAN;
     if (cz) *cz = o[7];
     if (cE) *cE = o[8];
     return HM_OK;
@@ -835,10 +886,13 @@ static hm_status run_batch(hm_ctx *ctx, int32_t nb, hm_bunch *const *bunches, co
         q.nbins = nbins ? nbins[i] : 0;
         HM_TRY(validate(&q.p, &q.o, q.b->n, q.nbins));
         if (q.o.timing) timing = true;
-        if (q.b->n == 0) {   // no wake (validated): E0 becomes 0/0 after an update (evolution.jl:426)
+        if (q.b->n == 0) {   // no wake (validated)
             hm_result &r = results[i];
             std::memset(&r, 0, sizeof(r));
-            r.E0 = (q.p.update_E0 && q.p.n_turns > 0) ? NAN : q.p.E0;
+            // MPI branch: E0 becomes 0/0 after an update (evolution.jl:426); the
+            // serial branch updates only a non-empty bunch (evolution.jl:437-447)
+            r.E0 = (q.p.update_E0 && q.p.n_turns > 0 && !q.o.serial) ? NAN : q.p.E0;
+            r.corr_z_dE = NAN;
             r.turns_done = q.p.n_turns;
             continue;
         }

@@ -126,6 +126,7 @@ struct RunBunch {
     BunchDev d{};
     bool active = false;      // still turning
     bool finished = false;    // epilogue done, results valid
+    bool fail_setup = false;  // test hook: the set-up fails by request after the slot is taken
 };
 
 // A set of device-resident bunches that advance together: every turn is one
@@ -137,7 +138,8 @@ class Batch {
     explicit Batch(hm_ctx *ctx) : ctx_(ctx) {}
     ~Batch();
     // set up new bunches (initial spreads, grids, tables, FFT plan); their
-    // slot indices are appended to idx
+    // slot indices are appended to idx.  All or nothing: on an error every
+    // slot this call took is released again and idx is left as it was.
     hm_status add(const std::vector<RunBunch> &specs, std::vector<int> *idx);
     // run up to `turns` turns of every active bunch; bunches that reach their
     // n_turns run the epilogue and become finished
@@ -155,6 +157,7 @@ class Batch {
     int64_t pass_count() const { return pass_count_; }
 
   private:
+    hm_status add_slots(const std::vector<RunBunch> &specs, std::vector<int32_t> &fresh);
     hm_status build_tables();
     hm_status upload_descriptors();
     hm_status run_subset_moments(const std::vector<int32_t> &which, int mode);

@@ -169,18 +169,26 @@ __device__ __forceinline__ void qe_pair_normals(uint64_t seed, uint64_t p, uint3
     box_muller(x, n0, n1);
 }
 
-// zero_alloc_interpolation! index and weight (wakefield.jl:38-42,64-78) for nb >= 2.
-// The clamp of z to [c0, c1] is one v_max_f64 and one v_min_f64; a NaN z
-// (which the reference rejects) takes cell 0 with weight 0, and its particle
-// still ends NaN through the RF kick.
-__device__ __forceinline__ void interp_index(double z, double c0, double c1, double inv, int nb,
-                                             int &k, double &w) {
+// zero_alloc_interpolation! index and weight (wakefield.jl:38-42,64-78) for
+// nb >= 2.  The clamp of z to [c0, c1] is one v_max_f64 and one v_min_f64,
+// which map a NaN z to c0; the reference's clamp keeps the NaN, its weight
+// u - floor(u) is NaN and so is the interpolated value (cell 0 for the index,
+// as the clamp of a converted NaN gives).  interp_index_pass leaves that
+// select out: in the turn pass a NaN-z particle's dE turns NaN through the RF
+// term of the same pass whatever its kick, so no result depends on it.
+__device__ __forceinline__ void interp_index_pass(double z, double c0, double c1, double inv,
+                                                  int nb, int &k, double &w) {
     const double cl = __builtin_fmin(__builtin_fmax(z, c0), c1);
     const double u = (cl - c0) * inv;
     const double fu = floor(u);
     const int kk = (fu == fu) ? (int)fu : 0;
     k = min(max(kk, 0), nb - 2);
     w = u - fu;
+}
+__device__ __forceinline__ void interp_index(double z, double c0, double c1, double inv, int nb,
+                                             int &k, double &w) {
+    interp_index_pass(z, c0, c1, inv, nb, k, w);
+    if (z != z) w = z;
 }
 
 // the interpolated value (wakefield.jl:94-102)
@@ -599,8 +607,8 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             double2 q0, q1;
             if (kick2) {
                 int k0, k1;
-                interp_index(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
-                interp_index(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
+                interp_index_pass(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
+                interp_index_pass(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
                 const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
                 // a full window, and any window of the all-flags pass (its
                 // histogram is in LDS, so its window has at least two entries),
@@ -1004,9 +1012,13 @@ __global__ __launch_bounds__(1024) void k_bin_reduce_whole(const BunchDev *__res
     if (threadIdx.x == 0) bin_scalars(B, t);
 }
 
+// g_i = lam_i * (cnt_i / n_h) (wakefield.jl:210,238).  A wake table by lag
+// (F_LAG) is convolved causally, P_k = I sum_j W_j g_{k+j}: the index
+// convolution of the reversed density, written back reversed (store_block).
 __device__ __forceinline__ double line_density(const BunchDev &B, const uint32_t *h, int i,
                                                double inv_nh) {
-    return B.lam[i] * ((double)h[i] * inv_nh);   // wakefield.jl:210,238
+    if (B.flags & F_LAG) i = B.nbins - 1 - i;
+    return B.lam[i] * ((double)h[i] * inv_nh);
 }
 
 // sX <- bit-reversed g_a + i g_{a-1}, zero-padded to F
@@ -1052,14 +1064,16 @@ __device__ inline void store_block(const BunchDev &B, const double2 *sX, int o, 
         if (i >= nb) continue;
         const double2 y0 = sX[bitrev(m, logF)];
         const double2 y1 = sX[bitrev(m + Bk, logF)];
-        put_potential(B, i, (y0.x + y1.y) * scale);
+        put_potential(B, (B.flags & F_LAG) ? nb - 1 - i : i, (y0.x + y1.y) * scale);
     }
 }
 
 __device__ inline void zero_block(const BunchDev &B, int o) {
     const int nb = B.nbins, Bk = B.B;
-    for (int m = threadIdx.x; m < Bk; m += blockDim.x)
-        if (o * Bk + m < nb) put_potential(B, o * Bk + m, 0.0);
+    for (int m = threadIdx.x; m < Bk; m += blockDim.x) {
+        const int i = o * Bk + m;
+        if (i < nb) put_potential(B, (B.flags & F_LAG) ? nb - 1 - i : i, 0.0);
+    }
 }
 
 // one workgroup per output block, wake support within one block (nJ <= 1)
@@ -1854,8 +1868,8 @@ __global__ __launch_bounds__(256) void k_pair_sums_final(int mode, const double 
             stats[1] = v[1] / (double)n;
         } else {
             const double vz = v[0] / (double)(n - 1), ve = v[1] / (double)(n - 1);
-            stats[2] = sqrt(vz > 0.0 ? vz : 0.0);
-            stats[3] = sqrt(ve > 0.0 ? ve : 0.0);
+            stats[2] = sqrt(jl_max(0.0, vz));   // compute_std: sqrt(max(0, variance))
+            stats[3] = sqrt(jl_max(0.0, ve));
             double r = v[2] / sqrt(v[0] * v[1]);   // Statistics.cor, clamped to [-1, 1]
             stats[4] = (r != r) ? r : (r > 1.0 ? 1.0 : (r < -1.0 ? -1.0 : r));
         }

@@ -95,6 +95,7 @@ static hm_status admit(hm_server *s) {
         q.o.wake_table = job.wake.empty() ? nullptr : job.wake.data();
         q.o.resonators = job.res.empty() ? nullptr : job.res.data();
         q.nbins = job.spec.nbins;
+        q.fail_setup = job.spec.fail_setup != 0;
         if (job.spec.fail_after_turns > 0 && job.spec.fail_after_turns < q.p.n_turns) {
             q.p.n_turns = job.spec.fail_after_turns;
             job.failed_by_request = true;
@@ -106,18 +107,28 @@ static hm_status admit(hm_server *s) {
     }
     if (specs.empty()) return HM_OK;
     std::vector<int> idx;
-    hm_status st = s->batch->add(specs, &idx);
-    if (st != HM_OK) {
-        for (auto &j : admitted) {
+    if (s->batch->add(specs, &idx) == HM_OK) {
+        for (size_t k = 0; k < admitted.size(); ++k) {
+            admitted[k].slot = idx[k];
+            s->resident.push_back(std::move(admitted[k]));
+        }
+        return HM_OK;
+    }
+    // Batch::add took nothing: admit the jobs one at a time, so that a job
+    // whose set-up fails (device memory, a failure by request) fails alone
+    for (size_t k = 0; k < admitted.size(); ++k) {
+        Job &j = admitted[k];
+        std::vector<int> one;
+        const hm_status st = s->batch->add({specs[k]}, &one);
+        if (st != HM_OK) {
             s->used -= j.bytes;
             hm_bunch_destroy(j.b);
+            j.b = nullptr;
             push_result(s, j, st, nullptr, 0, 0);
+            continue;
         }
-        return st;
-    }
-    for (size_t k = 0; k < admitted.size(); ++k) {
-        admitted[k].slot = idx[k];
-        s->resident.push_back(std::move(admitted[k]));
+        j.slot = one[0];
+        s->resident.push_back(std::move(j));
     }
     return HM_OK;
 }

@@ -40,6 +40,7 @@ enum BunchFlags : uint32_t {
     F_INJ = 32u,       // injected normals instead of Philox
     F_CHARGE = 64u,    // fixed wake current (charge extension / stage entry)
     F_GHIST = 128u,    // histogram straight to global memory (too many bins for LDS)
+    F_LAG = 256u,      // wake table by lag behind the source: bin k feels bins k+j (causal)
 };
 
 // the all-physics-on configuration the turn pass is specialised for, and the

@@ -26,9 +26,10 @@ struct HmOptions
     normals::Ptr{Float64}; seed::UInt64; turn_offset::Int64; wake_table::Ptr{Float64}
     delta_sigma::Float64; bin_half_width_sigmas::Float64; charge::Float64
     n_resonators::Int32; timing::Int32; resonators::Ptr{Float64}; fft_block::Int32; block_size::Int32
-    monitor::Ptr{Float64}
+    monitor::Ptr{Float64}; serial::Int32; fom_stats::Int32
 end
-HmOptions(; seed::Integer=0) = HmOptions(C_NULL, UInt64(seed), 0, C_NULL, 0.0, 0.0, 0.0, 0, 0, C_NULL, 0, 0, C_NULL)
+HmOptions(; seed::Integer=0, serial::Bool=false) =
+    HmOptions(C_NULL, UInt64(seed), 0, C_NULL, 0.0, 0.0, 0.0, 0, 0, C_NULL, 0, 0, C_NULL, Int32(serial), 0)
 
 struct HmResult
     sigma_E::Float64; sigma_z::Float64; E0::Float64; turns_done::Int64
@@ -64,14 +65,16 @@ end
 
 The reference's turn loop (src/evolution.jl:206-471) on the GPU.  `particles`
 is mutated in place; `length(buffers.λ)` gives the bin count.  One GPU holds
-the whole bunch, so `use_mpi` is accepted and ignored.
+the whole bunch, so the MPI branch's collectives are on-device reductions;
+`use_mpi = false` selects the serial branch (two-pass `compute_std` spreads,
+evolution.jl:245-253,461-467), which, as in the reference, cannot run the wake.
 """
 function longitudinal_evolve!(particles::StructArray{Particle{Float64}}, params::SimulationParameters,
                               buffers::SimulationBuffers{Float64}, dev::MI355X, use_mpi::Bool)
     z = particles.coordinates.z
     dE = particles.coordinates.ΔE
     p = Ref(HmParams(params))
-    o = Ref(HmOptions(seed=dev.seed))
+    o = Ref(HmOptions(seed=dev.seed, serial=!use_mpi))
     r = Ref{HmResult}()
     st = GC.@preserve z dE ccall((:hm_evolve, LIB), Int32,
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Ref{HmParams}, Ref{HmOptions}, Ref{HmResult}),

@@ -57,10 +57,16 @@ typedef struct {
     const double *normals;   /* injected QE normals, host, turn-major [n_turns][n]; NULL = Philox */
     uint64_t seed;           /* Philox4x32-10 key */
     int64_t turn_offset;     /* absolute index of the first turn (Philox counter) */
-    const double *wake_table;   /* W by bin lag, host [nbins]; NULL = wakefield.jl:115-118 at centres */
+    const double *wake_table;   /* W at lag j*step behind the source, host [nbins]; NULL =
+                                   wakefield.jl:115-118 at the centres (the reference's index
+                                   convolution).  A table by lag is applied causally: bin k
+                                   feels the bins k+j ahead of it (larger z is the head,
+                                   wakefield.jl:111-113), P_k = I sum_j W_j g_{k+j} */
     double delta_sigma;         /* line-density kernel width; 0 = 0.15 (wakefield.jl:204); <0 = none */
     double bin_half_width_sigmas;  /* 0 = 7.5 (evolution.jl:270) */
-    double charge;              /* >0: wake current = charge [C] (replaces evolution.jl:377-386) */
+    double charge;              /* >0: wake current = charge [C] (replaces evolution.jl:377-386);
+                                   the line density is then counts/n_h without the delta kernel
+                                   (HM_ERR_ARG with delta_sigma > 0) */
     int32_t n_resonators;       /* >0: W by lag = sum of broadband resonators at s = j*step */
     int32_t timing;             /* 1: time the turn-pass kernel with HIP events */
     const double *resonators;   /* host [n_resonators][3] = (R_s [Ohm], f_r [Hz], Q) */
@@ -71,6 +77,14 @@ typedef struct {
                                    the binned count -- the moments of the turn-by-turn states of
                                    BeamTurn / assign_to_turn! (data_structures.jl:52-54,
                                    utils.jl:42-50) */
+    int32_t serial;             /* 1: use_mpi = false (evolution.jl:245-253,437-447,461-467):
+                                   two-pass compute_std (utils.jl:531-544) for the initial and
+                                   final spreads, no E0 update of an empty bunch; with the wake
+                                   HM_ERR_ARG, as the reference raises a MethodError there
+                                   (evolution.jl:267 calls initialize_mpi_buffers! with nothing) */
+    int32_t fom_stats;          /* 1: the final two-pass means and Statistics.cor(z, dE) in
+                                   hm_result (mean_z, mean_dE, corr_z_dE: two more reads of the
+                                   bunch); 0: corr_z_dE is NaN and the means are one-pass */
 } hm_options;
 
 /* Return of longitudinal_evolve! (evolution.jl:470) plus diagnostics. */
@@ -87,9 +101,10 @@ typedef struct {
     int64_t pass_launches;  /* number of turn-pass launches timed */
     double sigma_E0, sigma_z0;  /* initial spreads (evolution.jl:245-253) */
     double bin_half_width;      /* z_width actually used [m] */
-    double mean_z, mean_dE;     /* final centroid (two-pass means) */
+    double mean_z, mean_dE;     /* final centroid */
     double corr_z_dE;           /* Statistics.cor of the final z, dE: the emittance figure
-                                   of merit (parameter_sensitivity/figures_of_merit.jl:36-47) */
+                                   of merit (parameter_sensitivity/figures_of_merit.jl:36-47);
+                                   NaN unless hm_options.fom_stats */
 } hm_result;
 
 typedef struct hm_ctx hm_ctx;       /* one GPU, one stream, cached work buffers */
@@ -196,7 +211,8 @@ typedef struct {
     uint64_t particle_seed;    /* initial Gaussian bunch from hm_bunch_generate */
     double sigma_z, sigma_E;   /* initial bunch shape */
     int32_t fail_after_turns;  /* >0: the job fails by request after this many turns (test hook) */
-    int32_t pad_;
+    int32_t fail_setup;        /* 1: the job's set-up inside the batch fails by request, after
+                                  its slot is taken (test hook of the admission's error path) */
 } hm_job;
 
 typedef struct {

@@ -301,6 +301,26 @@ void or_wake_potential(const int64_t *counts, int64_t nb, const double *wake_tab
     free(wr); free(wi); free(lr); free(li);
 }
 
+/* Lag-indexed wake tables (the resonator and wake-table extensions) give W at a
+ * distance j*step BEHIND the source.  Larger z is the head (wakefield.jl:111-113:
+ * "positive z is upstream (arriving earlier)"), so bin k feels the charge of the
+ * bins k+j >= k: P_k = I sum_j W_j g_{k+j}.  Reversing g and P turns this into the
+ * index convolution of or_wake_potential. */
+void or_wake_potential_lag(const int64_t *counts, int64_t nb, const double *wake_tab,
+                           const double *lam_tab, double current, double *P) {
+    if (nb <= 0) return;
+    int64_t *rc = (int64_t *)malloc(sizeof(int64_t) * (size_t)nb);
+    double *rl = (double *)malloc(sizeof(double) * (size_t)nb);
+    double *rp = (double *)malloc(sizeof(double) * (size_t)nb);
+    for (int64_t i = 0; i < nb; ++i) {
+        rc[i] = counts[nb - 1 - i];
+        rl[i] = lam_tab[nb - 1 - i];
+    }
+    or_wake_potential(rc, nb, wake_tab, rl, current, rp);
+    for (int64_t i = 0; i < nb; ++i) P[i] = rp[nb - 1 - i];
+    free(rc); free(rl); free(rp);
+}
+
 void or_wake_potential_direct(const int64_t *counts, int64_t nb, const double *wake_tab,
                               const double *lam_tab, double current, double *P) {
     int64_t nh = 0;
@@ -574,12 +594,19 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
     const double mass = p->mass, voltage = p->voltage, radius = p->radius;
     const double alpha_c = p->alpha_c, phis = p->phi_s, freq_rf = p->freq_rf;
 
-    double sigma_E0 = gstd_ranks(dE, ch, R, scratch);
-    double sigma_z0 = gstd_ranks(z, ch, R, scratch);
+    const int serial = o->serial != 0;
+    if (serial && p->use_wakefield) {  /* evolution.jl:267: initialize_mpi_buffers! needs an MPI.Comm */
+        free(ch); free(scratch); return -5;
+    }
+    if (o->charge > 0 && o->delta_sigma > 0) { free(ch); free(scratch); return -1; }
+    /* evolution.jl:245-253: compute_global_std (MPI) or compute_std (serial) */
+    double sigma_E0 = serial ? or_serial_std(dE, n) : gstd_ranks(dE, ch, R, scratch);
+    double sigma_z0 = serial ? or_serial_std(z, n) : gstd_ranks(z, ch, R, scratch);
 
     double *edges = NULL, *centers = NULL, *wtab = NULL, *ltab = NULL, *P = NULL, *pot = NULL;
     int64_t *counts = NULL, *rcounts = NULL;
     double e_first = 0, e_last = 0, inv_step = 0;
+    int lag = 0;   /* lag-indexed (causal) wake table */
     if (p->use_wakefield) {
         if (nbins <= 0) { free(ch); free(scratch); return -3; }      /* evolution.jl:264 */
         if (!o->wake_table && o->n_resonators <= 0 && p->pipe_radius <= 0) {
@@ -601,7 +628,10 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
         double wf = (((120.0 * OR_PI) * OR_SPEED_LIGHT) / (OR_PI * p->pipe_radius));
         double ws = sqrt(jl_max(0.0, (2.0 * 30.0) / p->pipe_radius));
         double inv_ctau = 1.0 / 4e-3;
-        double dsig = o->delta_sigma == 0.0 ? 0.15 : o->delta_sigma;
+        /* with a charge the line density is counts/n_h: the delta kernel belongs to the
+         * reference's dimensionless current heuristic */
+        double dsig = o->charge > 0 ? -1.0 : (o->delta_sigma == 0.0 ? 0.15 : o->delta_sigma);
+        lag = o->wake_table != NULL || o->n_resonators > 0;
         for (int64_t i = 0; i < nbins; ++i) {
             if (o->wake_table) {
                 wtab[i] = o->wake_table[i];
@@ -687,7 +717,10 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
                 for (int64_t b = 0; b < nbins; ++b) nh += counts[b];
                 o->monitor[4 * t + 3] = (double)nh;
             }
-            or_wake_potential(counts, nbins, wtab, ltab, current, P);
+            if (lag)
+                or_wake_potential_lag(counts, nbins, wtab, ltab, current, P);
+            else
+                or_wake_potential(counts, nbins, wtab, ltab, current, P);
 #pragma omp parallel for schedule(static, 1) num_threads(R)
             for (int r = 0; r < R; ++r) {
                 int64_t off = ch[r].off, cnt = ch[r].cnt;
@@ -696,7 +729,15 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
             }
         }
 
-        if (p->update_E0) { /* evolution.jl:404-435 */
+        if (p->update_E0 && serial) { /* evolution.jl:437-447: compute_mean_diff, n > 0 only */
+            if (n > 0) {
+                double s = 0.0;
+                for (int64_t i = 0; i < n; ++i) s += dE[i] - dE_init[i];
+                double mean = s / (double)n;
+                E0 = E0 + mean;
+                for (int64_t i = 0; i < n; ++i) dE[i] -= mean;
+            }
+        } else if (p->update_E0) { /* evolution.jl:404-435 */
 #pragma omp parallel for schedule(static, 1) num_threads(R)
             for (int r = 0; r < R; ++r) {
                 double s = 0.0;
@@ -714,8 +755,9 @@ int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *
         if (o->monitor) o->monitor[4 * t + 2] = E0;
     }
 
-    res->sigma_E = gstd_ranks(dE, ch, R, scratch);
-    res->sigma_z = gstd_ranks(z, ch, R, scratch);
+    /* evolution.jl:461-467 */
+    res->sigma_E = serial ? or_serial_std(dE, n) : gstd_ranks(dE, ch, R, scratch);
+    res->sigma_z = serial ? or_serial_std(z, n) : gstd_ranks(z, ch, R, scratch);
     res->E0 = E0;
     res->turns_done = p->n_turns;
     res->drift_skipped = drift_skipped;

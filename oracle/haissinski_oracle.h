@@ -38,12 +38,17 @@ typedef struct {
     int32_t nranks;         /* rank-equivalents (threads) for the MPI decomposition; <=0 -> 1 */
     int32_t pad_;
     /* Extensions (all off = the reference):                                    */
-    const double *wake_table;      /* W by bin lag j, [nbins]; NULL -> W(c_j) of wakefield.jl:194-196 */
+    const double *wake_table;      /* W by lag j*step behind the source, [nbins]; NULL -> W(c_j)
+                                      of wakefield.jl:194-196 */
     double delta_sigma;            /* smoothing width; 0 -> 0.15 (wakefield.jl:204); <0 -> no kernel */
     double bin_half_width_sigmas;  /* 0 -> 7.5 (evolution.jl:270) */
-    double charge;                 /* >0: current = charge [C] instead of evolution.jl:377-386 */
+    double charge;                 /* >0: current = charge [C] instead of evolution.jl:377-386, and
+                                      the line density is counts/n_h without the delta kernel
+                                      (delta_sigma > 0 together with charge is an error) */
     int32_t n_resonators;          /* >0: W by lag = sum of broadband resonators at s = j*step */
-    int32_t pad2_;
+    int32_t serial;                /* 1: use_mpi = false (evolution.jl:245-253,437-447,461-467):
+                                      two-pass compute_std spreads, no E0 update of an empty
+                                      bunch; with the wake the reference raises (-5) */
     const double *resonators;      /* [n_resonators][3] = (R_s [Ohm], f_r [Hz], Q) */
     double *monitor;               /* [n_turns][4] or NULL: z centroid and sigma_z after the drift
                                       (wake on), E0 after the turn, binned count */
@@ -95,6 +100,11 @@ void or_fft(double *re, double *im, int64_t L, int inverse);
  * wake_tab[j] = W(c_j) (wakefield.jl:194-196), lam_tab[i] = delta(c_i, 0.15) (wakefield.jl:219-221). */
 void or_wake_potential(const int64_t *counts, int64_t nb, const double *wake_tab,
                        const double *lam_tab, double current, double *P);
+/* Lag-indexed (causal) wake: P_k = current * sum_{j>=0} W_j g_{k+j}, the charge at
+ * bin k+j (ahead of bin k: larger z is the head, wakefield.jl:111-113) acting on
+ * bin k through W at lag j*step.  Same FFT convolution on reversed g and P. */
+void or_wake_potential_lag(const int64_t *counts, int64_t nb, const double *wake_tab,
+                           const double *lam_tab, double current, double *P);
 /* same linear convolution by the direct index sum (independent check) */
 void or_wake_potential_direct(const int64_t *counts, int64_t nb, const double *wake_tab,
                               const double *lam_tab, double current, double *P);
@@ -115,7 +125,9 @@ void or_generate_gaussian(uint64_t seed, double mu_z, double sigma_z, double mu_
 void or_generate_particles(uint64_t seed, double mu_z, double mu_E, double sigma_z, double sigma_E,
                            int64_t n, double *z, double *dE, double *sigma_E_out,
                            double *sigma_z_out, double *L_out);
-/* evolution.jl:206-471, MPI branch with nranks rank-equivalents */
+/* evolution.jl:206-471, MPI branch with nranks rank-equivalents (serial branch with
+ * o->serial).  Returns 0, or -1 bad options, -2 wake with no particles, -3 nbins <= 0,
+ * -4 pipe_radius <= 0, -5 wake with the serial branch (MethodError of evolution.jl:267). */
 int or_evolve(double *z, double *dE, int64_t n, int64_t nbins, const or_params *p,
               const or_options *o, or_result *res, double *P_last);
 

@@ -140,14 +140,33 @@ def wake_current(N, E0, radius, sz, eta0, sE0):
     return (((pf / E0) / ((2 * math.pi) * radius)) * sz) / (eta0 * (sE0 * sE0))
 
 
-def evolve(z, dE, nbins, p: dict, normals):
-    """All five flags as given in p; normals[t, i] are the injected QE draws."""
+def std2(x: np.ndarray) -> float:
+    """compute_std (utils.jl:531-544): two passes, n - 1 in the variance."""
+    n = x.size
+    if n <= 1:
+        return 0.0
+    s = 0.0
+    for v in x.tolist():
+        s += v
+    mu = s / n
+    acc = 0.0
+    for v in x.tolist():
+        d = v - mu
+        acc += d * d
+    return math.sqrt(jl_max(0.0, acc / (n - 1)))
+
+
+def evolve(z, dE, nbins, p: dict, normals, serial: bool = False):
+    """All five flags as given in p; normals[t, i] are the injected QE draws.
+    ``serial``: the use_mpi = false branch (no wake), two-pass spreads and no
+    E0 update of an empty bunch (evolution.jl:245-253,437-447,461-467)."""
     z = np.array(z, dtype=np.float64)
     dE = np.array(dE, dtype=np.float64)
     n = z.size
     E0 = p["E0"]
-    sE0 = gstd(dE)
-    sz0 = gstd(z)
+    spread = std2 if serial else gstd
+    sE0 = spread(dE)
+    sz0 = spread(z)
     if p["use_wakefield"]:
         zw = 7.5 * sz0
         if zw < 1e-9:
@@ -184,11 +203,11 @@ def evolve(z, dE, nbins, p: dict, normals):
             counts = histogram(z, edges[0], edges[-1], inv_step, nbins)
             P = potential_fft(counts, wtab, ltab, cur)
             dE = dE - interpolate(z, centers, P)
-        if p["update_E0"]:
+        if p["update_E0"] and not (serial and n == 0):
             s = 0.0
             for v in (dE - dE_init).tolist():
                 s += v
             mean = s / n
             E0 = E0 + mean
             dE = dE - mean
-    return z, dE, gstd(dE), gstd(z), E0
+    return z, dE, spread(dE), spread(z), E0

@@ -40,7 +40,7 @@ class OrOptions(C.Structure):
         ("normals", _dp), ("seed", C.c_uint64), ("turn_offset", C.c_int64),
         ("nranks", C.c_int32), ("pad_", C.c_int32), ("wake_table", _dp),
         ("delta_sigma", C.c_double), ("bin_half_width_sigmas", C.c_double),
-        ("charge", C.c_double), ("n_resonators", C.c_int32), ("pad2_", C.c_int32),
+        ("charge", C.c_double), ("n_resonators", C.c_int32), ("serial", C.c_int32),
         ("resonators", _dp), ("monitor", _dp),
     ]
 
@@ -92,6 +92,7 @@ def lib():
         L.or_fft.argtypes = [_dp, _dp, C.c_int64, C.c_int]
         L.or_wake_potential.argtypes = [_ip, C.c_int64, _dp, _dp, C.c_double, _dp]
         L.or_wake_potential_direct.argtypes = [_ip, C.c_int64, _dp, _dp, C.c_double, _dp]
+        L.or_wake_potential_lag.argtypes = [_ip, C.c_int64, _dp, _dp, C.c_double, _dp]
         L.or_wake_current.restype = C.c_double
         L.or_wake_current.argtypes = [C.c_int64] + [C.c_double] * 5
         L.or_interpolate.argtypes = [_dp, C.c_int64, _dp, C.c_int64, _dp, _dp]
@@ -206,11 +207,16 @@ def fft(re, im, inverse=False):
     return re, im
 
 
-def wake_potential(counts, wake_tab, lam_tab, current, direct=False):
+def wake_potential(counts, wake_tab, lam_tab, current, direct=False, lag=False):
+    """Index convolution P_k = I sum_j W_j g_{k-j} (the reference's, wakefield.jl:231-319),
+    or with ``lag`` the causal one of lag-indexed tables, P_k = I sum_j W_j g_{k+j}."""
     counts = np.ascontiguousarray(counts, dtype=np.int64)
     wake_tab = f64(wake_tab); lam_tab = f64(lam_tab)
     P = np.empty(counts.size)
-    fn = lib().or_wake_potential_direct if direct else lib().or_wake_potential
+    if lag:
+        fn = lib().or_wake_potential_lag
+    else:
+        fn = lib().or_wake_potential_direct if direct else lib().or_wake_potential
     fn(_i(counts), counts.size, _d(wake_tab), _d(lam_tab), current, _d(P))
     return P
 
@@ -275,7 +281,7 @@ class EvolveResult:
 
 def evolve(z, dE, nbins, params: dict, normals=None, seed=0, turn_offset=0, nranks=1,
            wake_table=None, delta_sigma=0.0, bin_half_width_sigmas=0.0, charge=0.0,
-           resonators=None, monitor=None):
+           resonators=None, monitor=None, serial=False):
     """Run the restated turn loop in place on copies; returns (z, dE, EvolveResult)."""
     z = f64(z).copy(); dE = f64(dE).copy()
     p = OrParams(**{k: params[k] for k, _ in OrParams._fields_})
@@ -301,6 +307,7 @@ def evolve(z, dE, nbins, params: dict, normals=None, seed=0, turn_offset=0, nran
         keep.append(rs)
         o.n_resonators = rs.size // 3
         o.resonators = _d(rs)
+    o.serial = int(bool(serial))
     if monitor is not None:   # float64 [n_turns, 4], filled in place
         assert monitor.dtype == np.float64 and monitor.size >= 4 * params["n_turns"]
         o.monitor = _d(monitor)

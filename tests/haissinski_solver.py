@@ -10,11 +10,13 @@ state.  In the smooth-motion limit that state is
 with K = coeff / rf the drift per unit energy (evolution.jl:157-164) and
 sigma_E the excitation level sigma_E0 (quantum_excitation.jl:58-59 balances
 synchrotron_radiation.jl:44-55 at sigma_E = sigma_E0).  The potential P is
-the reference's own: counts / n_h times the delta kernel, index-convolved
-with the wake table by lag and scaled by the current (wakefield.jl:186-319),
-here with a fixed current (the charge option) so the fixed point depends on
-lambda only.  The fixed point is found by damped iteration on the bin
-centres.
+the tracker's for a lag-indexed wake with a fixed current (the charge
+option, so the fixed point depends on lambda only): counts / n_h on the bin
+grid of wakefield.jl:186-319 (the delta kernel belongs to the reference's
+dimensionless current and is left out with a charge), each bin feeling the
+bins ahead of it (larger z is the head, wakefield.jl:111-113) through the
+wake at their distance.  The fixed point is found by damped iteration on the
+bin centres.
 """
 from __future__ import annotations
 
@@ -28,12 +30,12 @@ C_LIGHT = 299792458.0
 
 
 def solve(p: dict, nbins: int, z_width: float, resonators, current: float, sigma_E: float,
-          iters: int = 400, delta_sigma: float = 0.15):
+          iters: int = 400, delta_sigma: float | None = None):
     """Returns (centres, probability per bin, P at the centres)."""
     edges, c, step = O.bin_grid(z_width, nbins)
     wtab = np.array([sum(O.resonator_wake(j * step, *r) for r in resonators)
                      for j in range(nbins)])
-    ltab = np.array([O.delta(x, delta_sigma) for x in c])
+    ltab = np.ones(nbins) if delta_sigma is None else np.array([O.delta(x, delta_sigma) for x in c])
     g0 = p["E0"] / p["mass"]
     b0 = math.sqrt(1.0 - 1.0 / (g0 * g0))
     eta0 = p["alpha_c"] - 1.0 / (g0 * g0)
@@ -48,9 +50,9 @@ def solve(p: dict, nbins: int, z_width: float, resonators, current: float, sigma
     lam /= lam.sum()
     P = np.zeros(nbins)
     for _ in range(iters):
-        g = ltab * lam
+        g = (ltab * lam)[::-1]   # P_k = I sum_j W_j g_{k+j}: the index convolution reversed
         P = current * np.fft.irfft(w_hat * np.fft.rfft(np.concatenate([g, np.zeros(L - nbins)])),
-                                   L)[:nbins]
+                                   L)[:nbins][::-1]
         F = f_rf - P
         phi = -np.concatenate([[0.0], np.cumsum(0.5 * (F[1:] + F[:-1]) * step)])
         phi -= phi[i0]

@@ -24,11 +24,28 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert all(re.search(rf"\b{n}\s*\(", text) for n in names)
 
 
-def test_struct_layouts_match_header_sizes():
-    # hm_params: 3 doubles + int64 + 5 doubles + int64 + 5 int32 (+pad)
-    assert C.sizeof(_lib.HmParams) == 8 * 10 + 4 * 5 + 4
-    assert C.sizeof(_lib.HmResult) == 8 * 3 + 8 + 4 + 4 + 8 * 3 + 8 + 8 * 3 + 8 * 3
-    assert C.sizeof(_lib.HmOptions) == 8 + 8 + 8 + 8 + 8 * 3 + 4 + 4 + 8 + 4 + 4 + 8
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    # the header compiled by gcc (tests/abi_layout.c) against the ctypes binding:
+    # every struct's size and every field's offset
+    import os
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = tmp_path / "abi_layout"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-o", str(exe), os.path.join(here, "abi_layout.c")],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    binding = {"hm_params": _lib.HmParams, "hm_options": _lib.HmOptions, "hm_result": _lib.HmResult,
+               "hm_job": _lib.HmJob, "hm_job_result": _lib.HmJobResult}
+    seen = 0
+    for line in out.splitlines():
+        st, field, val = line.split()
+        cls = binding[st]
+        if field == "sizeof":
+            assert C.sizeof(cls) == int(val), st
+        else:
+            assert getattr(cls, field).offset == int(val), (st, field)
+        seen += 1
+    assert seen >= 60
 
 
 @pytest.mark.parametrize("zw,nb", [(0.03, 1024), (7.5 * 4.123456789e-3, 1000), (1e-6, 7),

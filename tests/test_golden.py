@@ -48,11 +48,15 @@ def test_gpu_matches_fixture(path):
                              charge=ext.get("charge", 0.0),
                              bin_half_width_sigmas=ext.get("bin_half_width_sigmas", 0.0),
                              delta_sigma=ext.get("delta_sigma", 0.0))
+    use_mpi = not ext.get("serial", False)
     sE, sz, E0 = H.longitudinal_evolve(part, H.SimulationParameters.from_dict(p),
                                        H.create_simulation_buffers(case["n"], max(case["nbins"], 1)),
-                                       H.MI355X.default(), True, opts)
+                                       H.MI355X.default(), use_mpi, opts)
     assert normwise(part.coordinates.z, d["z"]) <= COORD_RTOL
     assert normwise(part.coordinates.ΔE, d["dE"]) <= COORD_RTOL
     assert E0 == pytest.approx(d["scalars"][2], rel=1e-14)
-    assert sE == pytest.approx(d["scalars"][0], rel=1e-10)
-    assert sz == pytest.approx(d["scalars"][1], rel=1e-10)
+    # the serial branch's two-pass spreads are accurate to rounding, so they are
+    # held to 1e-13 (its one-pass counterpart differs by ~2e-11 on that fixture)
+    rtol = 1e-13 if not use_mpi else 1e-10
+    assert sE == pytest.approx(d["scalars"][0], rel=rtol)
+    assert sz == pytest.approx(d["scalars"][1], rel=rtol)

@@ -214,11 +214,20 @@ def test_pair_stats_match_numpy():
 def test_emittance_figure_of_merit():
     """compute_fom (figures_of_merit.jl:36-47) from the resident run's own
     correlation and from the downloaded particles agree with numpy."""
+    import dataclasses
     s = _specs()[0]
     b = _bunch(s)
-    r = b.evolve(_params(30, s["V"]), s["nbins"], _opts(s))
+    r0 = b.evolve(_params(30, s["V"]), s["nbins"], _opts(s))
+    assert np.isnan(r0.corr_z_dE)            # not asked for: no extra reads of the bunch
+    with pytest.raises(ValueError):
+        H.compute_fom(H.EmittanceFoM(), None, r0)
+    b.close()
+    b = _bunch(s)
+    r = b.evolve(_params(30, s["V"]), s["nbins"], dataclasses.replace(_opts(s), fom_stats=True))
+    assert (r.sigma_E, r.sigma_z, r.E0) == (r0.sigma_E, r0.sigma_z, r0.E0)
     z, e = b.download()
     b.close()
+    assert r.mean_z == pytest.approx(z.mean(), rel=1e-12)
     ref = r.sigma_z * r.sigma_E * np.sqrt(1 - np.corrcoef(z, e)[0, 1] ** 2)
     f1 = H.compute_fom(H.EmittanceFoM(), None, r)
     f2 = H.compute_fom(H.EmittanceFoM(), H.Particles(z, e), (r.sigma_E, r.sigma_z, r.E0))
@@ -237,11 +246,12 @@ def test_served_voltage_pair_is_a_common_random_number_difference():
     res = {r.job_id: r for r in srv.run(turns_per_tick=5)}
     srv.close()
     assert res[other].status == 0
+    import dataclasses
     alone = []
     for f in (1 - 0.005, 1 + 0.005):
         b = _bunch(s)
         p = _params(25, s["V"]).replace(voltage=s["V"] * f)
-        alone.append(b.evolve(p, s["nbins"], _opts(s)))
+        alone.append(b.evolve(p, s["nbins"], dataclasses.replace(_opts(s), fom_stats=True)))
         b.close()
     for jid, ra in zip((lo, hi), alone):
         assert (res[jid].result.sigma_E, res[jid].result.E0) == (ra.sigma_E, ra.E0)
@@ -249,3 +259,27 @@ def test_served_voltage_pair_is_a_common_random_number_difference():
         d = H.fd_sensitivity(fom, res[lo].result, res[hi].result, s["V"], 0.005)
         da = H.fd_sensitivity(fom, alone[0], alone[1], s["V"], 0.005)
         assert np.isfinite(d) and d == da
+
+
+def test_failed_admission_releases_its_slots():
+    """Two jobs admitted in one tick, the second failing its set-up inside the
+    batch after its slot is taken (the fail_setup hook): the batch gives every
+    slot of that admission back, the healthy job is admitted again alone and
+    ends with the bits it has alone; a later job runs normally."""
+    import dataclasses
+    s0, s1 = _specs()[0], _specs()[1]
+    srv = H.Server(budget_bytes=8 << 30, max_slots=8)
+    good = srv.submit(_job(s0, 20))
+    bad = srv.submit(dataclasses.replace(_job(s1, 20), fail_setup=True))
+    res = {r.job_id: r for r in srv.run(turns_per_tick=5)}
+    later = srv.submit(_job(s1, 20))
+    res.update({r.job_id: r for r in srv.run(turns_per_tick=5)})
+    srv.close()
+    assert res[bad].status == CANCELLED
+    assert res[good].status == 0 and res[later].status == 0
+    for jid, s in ((good, s0), (later, s1)):
+        b = _bunch(s)
+        ra = b.evolve(_params(20, s["V"], s["flags"]), s["nbins"], _opts(s))
+        b.close()
+        assert (res[jid].result.sigma_E, res[jid].result.sigma_z, res[jid].result.E0) == \
+               (ra.sigma_E, ra.sigma_z, ra.E0)

@@ -429,3 +429,105 @@ def test_evolve_error_cases():
     q = dict(p, use_wakefield=False)
     r = O.evolve(z0[:0], e0[:0], 16, q)[2]
     assert r.status == 0 and math.isnan(r.E0)             # 0/0 mean in the E0 update
+
+
+# --------------------------------------------------------------------------
+# The serial branch (use_mpi = false)
+# --------------------------------------------------------------------------
+
+def test_serial_branch_matches_numpy_and_differs_from_one_pass():
+    # evolution.jl:245-253,461-467: compute_std (two passes) instead of
+    # compute_global_std; a bunch 100 sigma_E off zero makes the difference
+    p = W.ring_3gev_lattice(n_turns=5, flags=(False, True, True, True, True))
+    n = 777
+    z0, e0 = O.generate_gaussian(9, 0.0, 4e-3, 3e8, 3e6, n)
+    nrm = np.vstack([O.normals(9, t, 0, n) for t in range(5)])
+    z1, e1, r = O.evolve(z0, e0, 0, p, normals=nrm, serial=True)
+    assert r.status == 0
+    z2, e2, sE, sz, E0 = NO.evolve(z0, e0, 0, p, nrm, serial=True)
+    assert np.max(np.abs(z1 - z2)) <= 1e-13 * np.sqrt(np.mean(z2 ** 2))
+    assert np.max(np.abs(e1 - e2)) <= 1e-13 * np.sqrt(np.mean(e2 ** 2))
+    assert r.E0 == pytest.approx(E0, rel=1e-15)
+    assert r.sigma_E == pytest.approx(sE, rel=1e-14)
+    assert r.sigma_z == pytest.approx(sz, rel=1e-14)
+    # the two-pass spread is the exact one (long double reference) ...
+    exact = float(np.sqrt(np.var(e1.astype(np.longdouble), ddof=1)))
+    assert r.sigma_E == pytest.approx(exact, rel=1e-14)
+    # ... and the MPI branch's one-pass spread is measurably off it
+    r_mpi = O.evolve(z0, e0, 0, p, normals=nrm)[2]
+    assert abs(O.global_std(e1) - r.sigma_E) > 1e-12 * r.sigma_E
+    assert r_mpi.sigma_E != r.sigma_E
+
+
+def test_serial_branch_edge_cases():
+    p = W.ring_3gev_lattice(n_turns=3, flags=(False, True, True, True, True))
+    e = np.empty(0)
+    r = O.evolve(e, e, 0, p, serial=True)[2]
+    assert r.status == 0 and r.E0 == p["E0"]        # evolution.jl:437-447: no update at n = 0
+    r = O.evolve(e, e, 0, p)[2]
+    assert r.status == 0 and math.isnan(r.E0)      # the MPI branch divides 0 by 0
+    z0, e0 = O.generate_gaussian(1, 0.0, 4e-3, 0.0, 3e6, 10)
+    q = dict(p, use_wakefield=True)
+    # evolution.jl:267: initialize_mpi_buffers! has no method for comm = nothing
+    assert O.evolve(z0, e0, 16, q, serial=True)[2].status == -5
+
+
+# --------------------------------------------------------------------------
+# Lag-indexed (causal) wakes and the charge option
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nb", [1, 2, 7, 64, 1000])
+def test_lag_wake_potential_is_the_causal_sum(nb):
+    rng = np.random.default_rng(100 + nb)
+    counts = rng.integers(0, 500, nb).astype(np.int64)
+    counts[0] += 1
+    wt = rng.normal(size=nb)
+    lt = rng.uniform(0.5, 2.0, nb)
+    cur = 2.5e-9
+    P = O.wake_potential(counts, wt, lt, cur, lag=True)
+    g = lt * (counts * (1.0 / counts.sum()))
+    # brute force: bin k feels bins k + j ahead of it through W_j
+    Pb = np.array([sum(wt[j] * g[k + j] for j in range(nb - k)) for k in range(nb)]) * cur
+    assert np.linalg.norm(P - Pb) <= 1e-13 * np.linalg.norm(Pb)
+
+
+def test_lag_wake_single_source_kicks_only_bins_behind():
+    nb, m = 64, 40
+    counts = np.zeros(nb, dtype=np.int64)
+    counts[m] = 1000
+    wt = np.array([O.resonator_wake(j * 1e-3, *W.RESONATOR_BB) for j in range(nb)])
+    P = O.wake_potential(counts, wt, np.ones(nb), 1e-9, lag=True)
+    # nothing ahead of the source (FFT rounding only)
+    assert np.all(np.abs(P[m + 1:]) <= 1e-15 * np.abs(P).max())
+    np.testing.assert_allclose(P[:m + 1], 1e-9 * wt[m::-1], rtol=1e-13, atol=1e-13 * abs(wt).max())
+    # the reference's own (index) convolution runs the other way: bins above the source
+    Pi = O.wake_potential(counts, wt, np.ones(nb), 1e-9)
+    assert np.all(np.abs(Pi[:m]) <= 1e-15 * np.abs(Pi).max())
+
+
+def test_charge_runs_use_counts_without_the_delta_kernel():
+    p = W.ring_3gev_lattice(n_turns=4)
+    n = 3000
+    z0, e0 = O.generate_gaussian(4, 0.0, 4e-3, 0.0, 3e6, n)
+    kw = dict(seed=5, resonators=[W.RESONATOR_BB], charge=3e-9, bin_half_width_sigmas=8.0)
+    a = O.evolve(z0, e0, 256, p, **kw)
+    b = O.evolve(z0, e0, 256, p, delta_sigma=-1.0, **kw)
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2].E0 == b[2].E0
+    assert O.evolve(z0, e0, 256, p, delta_sigma=0.15, **kw)[2].status == -1
+    # the energy the bunch loses in one turn is charge x the loss integral of
+    # the causal resonator wake over the binned line density (one turn with
+    # no RF voltage, SR or QE and dE = 0, so the drift leaves z in place and
+    # only the wake and the E0 update act)
+    p1 = W.ring_3gev_lattice(n_turns=1, flags=(True, True, True, False, False))
+    p1 = dict(p1, voltage=0.0)
+    z1, e1, r1 = O.evolve(z0, np.zeros(n), 256, p1, seed=5, resonators=[W.RESONATOR_BB], charge=3e-9,
+                          bin_half_width_sigmas=8.0)
+    sz0 = O.global_std(z0)
+    edges, centres, step = O.bin_grid(8.0 * sz0, 256)
+    cnt = np.histogram(z0, bins=edges)[0].astype(float)
+    g = cnt / cnt.sum()
+    wt = np.array([O.resonator_wake(j * step, *W.RESONATOR_BB) for j in range(256)])
+    Pk = 3e-9 * np.array([np.dot(wt[:256 - k], g[k:]) for k in range(256)])
+    loss = -np.dot(np.interp(z0, centres, Pk), np.ones(n)) / n
+    assert r1.E0 - p1["E0"] == pytest.approx(loss, rel=1e-6)
