@@ -12,8 +12,9 @@ Output: one JSON line.  ``roofline`` prices the dominant kernel (the turn
 pass) at 32 algorithmic bytes per particle-turn (read and write z and dE)
 against the 8 TB/s HBM peak, with its duration from HIP events on the
 library's stream; ``cpu_baseline`` is the oracle (the C restatement of the
-reference loop, OpenMP rank-equivalents) timed on a bounded sample on the
-host cores.
+reference loop, OpenMP rank-equivalents) timed on the host cores with the
+reference's method (median of fresh-copy runs after a warm-up turn) on the
+same bunch, bins and wake, at P = all available cores (up to 16) and P = 1.
 """
 from __future__ import annotations
 
@@ -58,22 +59,81 @@ def profiled_traffic(kernel: str, workload: str):
     return None, None
 
 
-def cpu_baseline(wl: dict, n: int, turns: int) -> dict:
+def host_cpu() -> tuple[str, int, int]:
+    """(CPU model, physical cores of the host, logical CPUs this process may use)."""
+    model, phys = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    cid = v
+                elif not k and pid is not None:
+                    phys.add((pid, cid))
+                    pid = cid = None
+            if pid is not None:
+                phys.add((pid, cid))
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, len(phys) or (os.cpu_count() or 1), avail
+
+
+def cpu_baseline(wl: dict, z0, e0, samples_all: int, turns_all: int, samples_one: int,
+                 turns_one: int) -> dict:
+    """The reference's timing method (tests/benchmark_total.jl:347-375): a
+    one-turn warm-up, then the median of several runs, each from a fresh copy
+    of the initial bunch, on the full workload shape (the bunch, bins and
+    wake the GPU runs), at P = the cores this process may use (up to 16,
+    the GPU's host share) and at P = 1.  The timed code is the oracle, the C
+    restatement of the reference loop (MPI branch, one OpenMP thread per rank
+    equivalent), not the Julia reference, which is not installed."""
+    import numpy as np
     from haissinskijulia_amd import workloads as W
     from oracle import oracle as O
-    cores = max(1, min(16, os.cpu_count() or 1))
-    p = W.ring_3gev_lattice(n_turns=turns)
-    z, e = O.generate_gaussian(wl["seed"], 0.0, W.SIGMA_Z, 0.0, W.SIGMA_E, n)
-    t0 = time.perf_counter()
-    _, _, r = O.evolve(z, e, wl["nbins"], p, seed=wl["seed"], nranks=cores,
-                       resonators=wl["resonators"], charge=wl["charge"],
-                       bin_half_width_sigmas=wl["half_width"])
-    dt = time.perf_counter() - t0
-    assert r.status == 0
-    return {"value": n * turns / dt, "unit": "particle-turns/s", "cores": cores, "kind": "port",
-            "sample": f"{n:.0e} particles x {turns} turns of the same lattice, bins and wake; "
-                      f"oracle C restatement of the reference loop (MPI branch), "
-                      f"{cores} OpenMP rank-equivalents, {dt:.1f} s"}
+    model, phys, avail = host_cpu()
+    cores = max(1, min(16, avail))
+    n = z0.size
+    kw = dict(resonators=wl["resonators"], charge=wl["charge"], bin_half_width_sigmas=wl["half_width"])
+
+    def timed(turns, ranks):
+        p = W.ring_3gev_lattice(n_turns=turns)
+        t0 = time.perf_counter()
+        _, _, r = O.evolve(z0, e0, wl["nbins"], p, seed=wl["seed"], nranks=ranks, **kw)
+        dt = time.perf_counter() - t0
+        assert r.status == 0
+        return dt
+
+    timed(1, cores)   # warm-up turn (benchmark_total.jl:265-271)
+    t_all = [timed(turns_all, cores) for _ in range(samples_all)]
+    log(f"cpu baseline: P={cores}: {[round(t, 2) for t in t_all]} s for {turns_all} turns")
+    t_one = [timed(turns_one, 1) for _ in range(samples_one)] if samples_one > 0 else []
+    if t_one:
+        log(f"cpu baseline: P=1: {[round(t, 2) for t in t_one]} s for {turns_one} turn(s)")
+    med_all = float(np.median(t_all))
+    out = {"value": n * turns_all / med_all, "unit": "particle-turns/s", "cores": cores,
+           "kind": "port",
+           "sample": f"the full workload shape ({n:.0e} particles, {wl['nbins']} bins, the same "
+                     f"wake and lattice): median of {samples_all} runs of {turns_all} turns at "
+                     f"P = {cores}, each from a fresh copy of the initial bunch after a one-turn "
+                     f"warm-up (tests/benchmark_total.jl:347-375); oracle C restatement of the "
+                     f"reference loop (MPI branch), not the Julia reference; each run's time "
+                     f"includes its set-up (initial spreads, tables), as the reference's does",
+           "cpu_model": model, "host_physical_cores": phys, "host_cpus_available": avail}
+    if t_one:
+        med_one = float(np.median(t_one))
+        out["p1_value"] = n * turns_one / med_one
+        out["p1_sample"] = f"median of {samples_one} runs of {turns_one} turn(s) at P = 1"
+    return out
 
 
 def _pct(x, q):
@@ -213,8 +273,12 @@ def main() -> int:
     ap.add_argument("--workload", default="ring3_1e8")
     ap.add_argument("--timing-turns", type=int, default=200,
                     help="turns of the separate run that times each kernel with HIP events")
-    ap.add_argument("--cpu-particles", type=int, default=4_000_000)
-    ap.add_argument("--cpu-turns", type=int, default=60)
+    ap.add_argument("--cpu-samples", type=int, default=5,
+                    help="fresh-copy runs of the CPU baseline at P = all cores (median)")
+    ap.add_argument("--cpu-turns", type=int, default=2, help="turns of each such run")
+    ap.add_argument("--cpu-samples-p1", type=int, default=3,
+                    help="fresh-copy runs at P = 1 (median); 0 skips them")
+    ap.add_argument("--cpu-turns-p1", type=int, default=1, help="turns of each P = 1 run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scan-turns", type=int, default=200,
                     help="turns of the 256-bunch scan (batched vs plain); 0 skips it")
@@ -245,6 +309,8 @@ def main() -> int:
     log(f"device {info}; workload {args.workload}: n={n} nbins={nbins}")
     bunch = H.DeviceBunch(n, dev)
     bunch.generate(wl["seed"], W.SIGMA_Z, W.SIGMA_E)
+    # the initial bunch for the CPU baseline: the same particles
+    cpu_bunch = None if args.no_cpu_baseline else bunch.download()
 
     def params(turns):
         return H.SimulationParameters.from_dict(W.ring_3gev_lattice(n_turns=turns))
@@ -315,7 +381,8 @@ def main() -> int:
             f"{sv['plain_p50_s']:.3f} s p99 {sv['plain_p99_s']:.3f} s")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_particles, args.cpu_turns)
+        out["cpu_baseline"] = cpu_baseline(wl, *cpu_bunch, args.cpu_samples, args.cpu_turns,
+                                           args.cpu_samples_p1, args.cpu_turns_p1)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} particle-turns/s")
     else:
         out["cpu_baseline"] = None

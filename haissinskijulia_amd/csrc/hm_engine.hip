@@ -107,26 +107,6 @@ static double ref_delta(double x, double sigma) {
     return s_inv * std::exp((x * x) * ef);
 }
 
-// broadband resonator wake [V/C] at distance s >= 0 behind the source (Chao,
-// Physics of Collective Beam Instabilities, eq. 2.84); W(0) is half the jump
-static double resonator_wake(double s, double Rs, double fr, double Q) {
-    const double wr = kTwoPi * fr;
-    const double a = wr / (2.0 * Q);
-    if (s < 0.0) return 0.0;
-    if (s == 0.0) return a * Rs;
-    const double tau = s / kSpeedLight;
-    const double env = 2.0 * a * Rs * std::exp(-a * tau);
-    const double d = wr * wr - a * a;
-    if (d > 0.0) {
-        const double wb = std::sqrt(d);
-        return env * (std::cos(wb * tau) - (a / wb) * std::sin(wb * tau));
-    } else if (d < 0.0) {
-        const double wb = std::sqrt(-d);
-        return env * (std::cosh(wb * tau) - (a / wb) * std::sinh(wb * tau));
-    }
-    return env * (1.0 - a * tau);
-}
-
 // evolution.jl:377-383
 double particle_factor(int64_t N) {
     static const double p10[11] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10};
@@ -585,42 +565,59 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         const double hw = q.o.bin_half_width_sigmas > 0 ? q.o.bin_half_width_sigmas : 7.5;
         double zw = hw * sz0;
         if (zw < 1e-9) zw = 1e-6;
-        std::vector<double> centers((size_t)nb);
-        double step;
-        bin_grid(zw, nb, nullptr, centers.data(), &step);
-        d.e_first = -zw;
-        d.e_last = zw;
-        d.inv_step = 1.0 / step;                                              // utils.jl:378
-        d.c_first = centers[0];
-        d.c_last = centers[(size_t)nb - 1];
-        d.inv_s = nb > 1 ? 1.0 / ((d.c_last - d.c_first) / (double)(nb - 1)) : 0.0;  // wakefield.jl:39-42
-        std::vector<double> W((size_t)nb, 0.0), lam((size_t)nb);
-        const double wf = ((120.0 * kPi) * kSpeedLight) / (kPi * q.p.pipe_radius);   // evolution.jl:279
-        const double ws = std::sqrt(jl_max_h(0.0, (2.0 * 30.0) / q.p.pipe_radius));  // evolution.jl:280
-        const double inv_ctau = 1.0 / 4e-3;                                          // wakefield.jl:158
         // with a charge the line density is counts/n_h: the delta kernel
         // belongs to the reference's dimensionless current heuristic
         const double dsig = q.o.charge > 0 ? -1.0 : (q.o.delta_sigma == 0.0 ? 0.15 : q.o.delta_sigma);
-        for (int64_t k = 0; k < nb; ++k) {
-            double wv;
-            if (!q.wake_copy.empty()) {
-                wv = q.wake_copy[(size_t)k];
-            } else if (!q.res_copy.empty()) {
-                const double sk = (double)k * step;
-                wv = 0.0;
-                for (size_t r = 0; r + 2 < q.res_copy.size(); r += 3)
-                    wv += resonator_wake(sk, q.res_copy[r], q.res_copy[r + 1], q.res_copy[r + 2]);
-            } else {
-                wv = ref_wake(centers[(size_t)k], wf, ws, inv_ctau);
-            }
-            W[(size_t)k] = wv;
+        // every centre only where a table is a function of the centres (the
+        // reference's wake W(c_j), the delta kernel); else the two ends
+        const bool lag_table = !q.wake_copy.empty() || !q.res_copy.empty();
+        const bool need_centres = !lag_table || dsig > 0;
+        std::vector<double> centers(need_centres ? (size_t)nb : 0);
+        double step;
+        if (need_centres) {
+            bin_grid(zw, nb, nullptr, centers.data(), &step);
+            d.c_first = centers[0];
+            d.c_last = centers[(size_t)nb - 1];
+        } else {
+            bin_grid(zw, nb, nullptr, nullptr, &step);
+            d.c_first = rn_ratio(zw, 1 - nb, nb);
+            d.c_last = rn_ratio(zw, nb - 1, nb);
+        }
+        d.e_first = -zw;
+        d.e_last = zw;
+        d.inv_step = 1.0 / step;                                              // utils.jl:378
+        d.inv_s = nb > 1 ? 1.0 / ((d.c_last - d.c_first) / (double)(nb - 1)) : 0.0;  // wakefield.jl:39-42
+        std::vector<double> W, lam((size_t)nb);
+        const double wf = ((120.0 * kPi) * kSpeedLight) / (kPi * q.p.pipe_radius);   // evolution.jl:279
+        const double ws = std::sqrt(jl_max_h(0.0, (2.0 * 30.0) / q.p.pipe_radius));  // evolution.jl:280
+        const double inv_ctau = 1.0 / 4e-3;                                          // wakefield.jl:158
+        for (int64_t k = 0; k < nb; ++k)
             lam[(size_t)k] = dsig > 0 ? ref_delta(centers[(size_t)k], dsig) : 1.0;
+        // the wake table: the caller's, the reference's W(c_j) on the host, or
+        // the resonators' W(j step) computed on the device (k_resonator_table;
+        // a resonator's wake has no zero tail on any grid here: support nb)
+        int64_t nw = nb;
+        ResonatorSet rset{};
+        if (!q.res_copy.empty()) {
+            rset.n = (int32_t)(q.res_copy.size() / 3);
+            if (rset.n > kMaxResonators)
+                return fail(HM_ERR_LIMIT, "at most %d resonators", kMaxResonators);
+            for (int r = 0; r < rset.n; ++r) {
+                rset.Rs[r] = q.res_copy[(size_t)(3 * r)];
+                rset.fr[r] = q.res_copy[(size_t)(3 * r + 1)];
+                rset.Q[r] = q.res_copy[(size_t)(3 * r + 2)];
+            }
+        } else {
+            W.assign((size_t)nb, 0.0);
+            for (int64_t k = 0; k < nb; ++k)
+                W[(size_t)k] = q.wake_copy.empty() ? ref_wake(centers[(size_t)k], wf, ws, inv_ctau)
+                                                   : q.wake_copy[(size_t)k];
+            nw = 0;
+            for (int64_t k = nb - 1; k >= 0; --k)
+                if (W[(size_t)k] != 0.0) { nw = k + 1; break; }
         }
         q.wake_copy.clear();
         q.res_copy.clear();
-        int64_t nw = 0;
-        for (int64_t k = nb - 1; k >= 0; --k)
-            if (W[(size_t)k] != 0.0) { nw = k + 1; break; }
         const int maxB = max_fft_block(ctx_);
         if (q.o.fft_block > maxB)
             return fail(HM_ERR_LIMIT, "fft_block %d exceeds the LDS limit (%d)", q.o.fft_block, maxB);
@@ -656,7 +653,10 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
             }
         }
         HIP_TRY(hipMemcpyAsync(w.lam.p, lam.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(w.Wtmp.p, W.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
+        if (rset.n > 0)
+            HIP_TRY(launch_resonator_table(w.Wtmp.p, (int)nb, step, rset, s));
+        else
+            HIP_TRY(hipMemcpyAsync(w.Wtmp.p, W.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemsetAsync(w.P.p, 0, (size_t)nb * 8, s));
         HIP_TRY(hipMemsetAsync(w.P2.p, 0, (size_t)nb * 16, s));
         d.P2 = w.P2.p;
@@ -800,8 +800,14 @@ hm_status Batch::advance(int64_t turns) {
         int64_t left = turns - step;
         for (size_t i = 0; i < rb_.size(); ++i)
             if (used_[i] && rb_[i].active) left = std::min(left, rb_[i].n_turns - rb_[i].turns_done);
+        // Graphs pay off where the turn is short enough for the launches to
+        // matter; a large bunch's turn hides them, and its call skips the
+        // capture and instantiation
+        int64_t active_particles = 0;
+        for (size_t i = 0; i < rb_.size(); ++i)
+            if (used_[i] && rb_[i].active) active_particles += rb_[i].d.n;
         int64_t did = 1;
-        if (!timing_ && left >= kGraphTurns) {
+        if (!timing_ && left >= kGraphTurns && active_particles < kGraphMaxParticles) {
             if (!graph_exec_) HM_TRY(capture_graph());
             HIP_TRY(hipGraphLaunch(graph_exec_, ctx_->stream));
             did = kGraphTurns;

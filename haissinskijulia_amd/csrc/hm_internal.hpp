@@ -168,6 +168,9 @@ class Batch {
     void drop_graph();
 
     static constexpr int kGraphTurns = 16;   // turns per captured graph
+    // active particles from which turns are launched directly: a turn of
+    // 4e6 particles takes ~60 us on the GPU, ten times its launches' cost
+    static constexpr int64_t kGraphMaxParticles = 4'000'000;
     hipGraphExec_t graph_exec_ = nullptr;    // kGraphTurns turns of the current tables
 
     hm_ctx *ctx_;

@@ -1215,6 +1215,39 @@ __global__ __launch_bounds__(kConvInvThreads) void k_conv_inv(const BunchDev *__
     store_block(B, sX, o, cur);
 }
 
+// Broadband resonator wake [V/C] at distance s >= 0 behind the source
+// (Chao, Physics of Collective Beam Instabilities, eq. 2.84):
+// W(s) = 2 a Rs e^{-a s/c} [cos(wb s/c) - (a/wb) sin(wb s/c)], a = w_r/(2Q),
+// wb = sqrt(w_r^2 - a^2); W(0) = a Rs, half the jump.  Overdamped (Q < 1/2)
+// and critical forms by cosh/sinh and the limit.
+__device__ inline double resonator_wake(double s, double Rs, double fr, double Q) {
+    const double wr = kTwoPi * fr;
+    const double a = wr / (2.0 * Q);
+    if (s < 0.0) return 0.0;
+    if (s == 0.0) return a * Rs;
+    const double tau = s / kSpeedLight;
+    const double env = 2.0 * a * Rs * exp(-a * tau);
+    const double d = wr * wr - a * a;
+    if (d > 0.0) {
+        const double wb = sqrt(d);
+        return env * (cos(wb * tau) - (a / wb) * sin(wb * tau));
+    } else if (d < 0.0) {
+        const double wb = sqrt(-d);
+        return env * (cosh(wb * tau) - (a / wb) * sinh(wb * tau));
+    }
+    return env * (1.0 - a * tau);
+}
+
+// the resonators' wake table by lag (set-up of an evolve call)
+__global__ void k_resonator_table(double *__restrict__ W, int nb, double step, ResonatorSet rs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    const double s = (double)j * step;
+    double w = 0.0;
+    for (int r = 0; r < rs.n; ++r) w += resonator_wake(s, rs.Rs[r], rs.fr[r], rs.Q[r]);
+    W[j] = w;
+}
+
 // FFT of one wake block per workgroup (set-up of an evolve call)
 __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__restrict__ bd,
                                                            const double *__restrict__ W) {
@@ -2007,6 +2040,13 @@ hipError_t launch_bin_reduce_whole(const BunchDev *bds, const int32_t *list, int
 hipError_t launch_finalize_whole(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
     if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize_whole, dim3(nb), dim3(1024), 0, s, bds, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_resonator_table(double *W, int nb, double step, const ResonatorSet &rs,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_resonator_table, dim3(blocks_for(nb, 256)), dim3(256), 0, s, W, nb, step,
+                       rs);
     return hipGetLastError();
 }
 

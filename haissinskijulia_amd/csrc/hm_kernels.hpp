@@ -90,6 +90,16 @@ hipError_t launch_bin_reduce_whole(const BunchDev *bds, const int32_t *bunch_lis
                                    hipStream_t s);
 hipError_t launch_finalize_whole(const BunchDev *bds, const int32_t *bunch_list, int nb,
                                  hipStream_t s);
+// Broadband resonators of a lag-indexed wake table, (R_s [Ohm], f_r [Hz], Q)
+// each, passed by value.
+constexpr int kMaxResonators = 8;
+struct ResonatorSet {
+    int32_t n;
+    double Rs[kMaxResonators], fr[kMaxResonators], Q[kMaxResonators];
+};
+// W[j] = sum of the resonators' wakes at s = j * step, j < nb
+hipError_t launch_resonator_table(double *W, int nb, double step, const ResonatorSet &rs,
+                                  hipStream_t s);
 // FFT of the wake blocks: What[j] = FFT_F(W[jB .. jB+B) zero-padded)
 hipError_t launch_fft_wake(const BunchDev *bd_dev, const double *W, int nJ, int F,
                            hipStream_t s);
