@@ -141,11 +141,12 @@ def _pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=float), q)) if len(x) else None
 
 
-def bench_scan(dev, turns: int) -> dict:
+def bench_scan(dev, turns: int, with_plain: bool = True) -> dict:
     """256 uneven bunches (a parameter scan) advanced together by
     hm_batch_evolve, against the plain path: the same bunches evolved one
-    after another.  All bunches are submitted at t = 0, so a bunch's latency
-    is its completion time."""
+    after another (skipped with ``with_plain`` false, e.g. under a profiler).
+    All bunches are submitted at t = 0, so a bunch's latency is its
+    completion time."""
     import haissinskijulia_amd as H
     from haissinskijulia_amd import workloads as W
     specs = W.scan_bunches()
@@ -158,7 +159,7 @@ def bench_scan(dev, turns: int) -> dict:
                                  bin_half_width_sigmas=s["half_width"])
 
     plain, done = [], 0.0
-    for s in specs:
+    for s in (specs if with_plain else []):
         b = H.DeviceBunch(s["n"], dev)
         b.generate(s["seed"], W.SIGMA_Z, W.SIGMA_E)
         t0 = time.perf_counter()
@@ -181,20 +182,21 @@ def bench_scan(dev, turns: int) -> dict:
     # never compares equal
     def bits(r):
         return struct.pack("<3d", r.sigma_E, r.sigma_z, r.E0)
-    same = all(bits(a) == bits(b) for a, (_, b) in zip(rs, plain))
     nonfinite = sum(1 for a in rs if not all(math.isfinite(x) for x in (a.sigma_E, a.sigma_z, a.E0)))
     work = sum(s["n"] for s in specs) * turns
-    lat_plain = [t for t, _ in plain]
-    return {"workload": f"{len(specs)} bunches, N log-uniform [1e4, 4e6] (sum {work // turns:.3g}), "
-                        f"V U[2.5, 4.0] MV, q U[0.5, 10] nC, 4096 bins, {turns} turns, "
-                        "seeds from SeedSequence(3)",
-            "batched_s": t_batch, "plain_s": lat_plain[-1], "speedup": lat_plain[-1] / t_batch,
-            "batched_particle_turns_per_s": work / t_batch,
-            "plain_particle_turns_per_s": work / lat_plain[-1],
-            "p50_s": t_batch, "p99_s": t_batch,
-            "plain_p50_s": _pct(lat_plain, 50), "plain_p99_s": _pct(lat_plain, 99),
-            "results_identical_to_plain": same,
-            "bunches_ending_non_finite": nonfinite}
+    out = {"workload": f"{len(specs)} bunches, N log-uniform [1e4, 4e6] (sum {work // turns:.3g}), "
+                       f"V U[2.5, 4.0] MV, q U[0.5, 10] nC, 4096 bins, {turns} turns, "
+                       "seeds from SeedSequence(3)",
+           "batched_s": t_batch, "batched_particle_turns_per_s": work / t_batch,
+           "p50_s": t_batch, "p99_s": t_batch, "bunches_ending_non_finite": nonfinite}
+    if plain:
+        lat_plain = [t for t, _ in plain]
+        out.update({"plain_s": lat_plain[-1], "speedup": lat_plain[-1] / t_batch,
+                    "plain_particle_turns_per_s": work / lat_plain[-1],
+                    "plain_p50_s": _pct(lat_plain, 50), "plain_p99_s": _pct(lat_plain, 99),
+                    "results_identical_to_plain": all(bits(a) == bits(b)
+                                                      for a, (_, b) in zip(rs, plain))})
+    return out
 
 
 def bench_serve(dev, count: int, rate: float, turns_per_tick: int, n_max: float = 1e6,
@@ -265,48 +267,11 @@ def bench_serve(dev, count: int, rate: float, turns_per_tick: int, n_max: float 
             "offered_particle_turns_per_s": work / jobs[-1]["arrival"]}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000, help="timed turns")
-    ap.add_argument("--warmup", type=int, default=20, help="untimed turns first")
-    ap.add_argument("--workload", default="ring3_1e8")
-    ap.add_argument("--timing-turns", type=int, default=200,
-                    help="turns of the separate run that times each kernel with HIP events")
-    ap.add_argument("--cpu-samples", type=int, default=5,
-                    help="fresh-copy runs of the CPU baseline at P = all cores (median)")
-    ap.add_argument("--cpu-turns", type=int, default=2, help="turns of each such run")
-    ap.add_argument("--cpu-samples-p1", type=int, default=3,
-                    help="fresh-copy runs at P = 1 (median); 0 skips them")
-    ap.add_argument("--cpu-turns-p1", type=int, default=1, help="turns of each P = 1 run")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--scan-turns", type=int, default=200,
-                    help="turns of the 256-bunch scan (batched vs plain); 0 skips it")
-    ap.add_argument("--serve-jobs", type=int, default=300,
-                    help="jobs of the serving run (server vs plain FIFO); 0 skips it")
-    ap.add_argument("--serve-rate", type=float, default=150.0, help="job arrivals per second")
-    ap.add_argument("--serve-tick", type=int, default=25, help="turns per server tick")
-    ap.add_argument("--serve-n-max", type=float, default=1e6,
-                    help="largest particle count of a served job (the full stream: 1e7)")
-    ap.add_argument("--serve-turn-range", default="100,2000",
-                    help="turn counts of served jobs, lo,hi (the full stream: 1000,20000)")
-    args = ap.parse_args()
-
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the turn loop's state of one bunch "
-                                   "lives in one GPU's HBM (no multi-GPU path)",
-                          "n_gpus": args.gpus}), flush=True)
-        return 2
-
-    import haissinskijulia_amd as H
-    from haissinskijulia_amd import workloads as W
-
-    wl = W.WORKLOADS[args.workload]
+def bench_main(H, W, dev, args, wl: dict, out: dict):
+    """The single-bunch leg: warm-up, the timed evolve call (the JSON line's
+    value), a separately timed run for the pass kernel's roofline, and the
+    plain-copy ceiling.  Returns the initial bunch for the CPU baseline."""
     n, nbins = wl["n"], wl["nbins"]
-    dev = H.MI355X(0)
-    info = dev.info()
-    log(f"device {info}; workload {args.workload}: n={n} nbins={nbins}")
     bunch = H.DeviceBunch(n, dev)
     bunch.generate(wl["seed"], W.SIGMA_Z, W.SIGMA_E)
     # the initial bunch for the CPU baseline: the same particles
@@ -338,19 +303,9 @@ def main() -> int:
     log(f"plain copy of {16 * n / 1e9:.1f} GB: {copy_ms:.3f} ms -> {copy_gbs:.0f} GB/s")
     traffic, traffic_src = profiled_traffic("k_turn_pass", args.workload)
 
-    out = {
-        "metric": METRIC,
+    out.update({
         "value": value,
-        "unit": "particle-turns/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
         "config": {"workload": args.workload, "particles": n, "nbins": nbins, "turns": args.steps,
                    "bin_half_width_sigmas": wl["half_width"], "charge_C": wl["charge"],
                    "wake": "broadband resonators " + ", ".join(f"({a:g} Ohm, {b:g} Hz, Q={c:g})"
@@ -365,12 +320,69 @@ def main() -> int:
         "whole_loop_frac_of_hbm_peak": value * BYTES_PER_PARTICLE_TURN / 1e9 / HBM_PEAK_GBS,
         "stream_copy_GBps": copy_gbs,
         "conv_ms": rt.ms_conv_avg,
-    }
+    })
+    return cpu_bunch
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000, help="timed turns")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed turns first")
+    ap.add_argument("--workload", default="ring3_1e8")
+    ap.add_argument("--timing-turns", type=int, default=200,
+                    help="turns of the separate run that times each kernel with HIP events")
+    ap.add_argument("--cpu-samples", type=int, default=5,
+                    help="fresh-copy runs of the CPU baseline at P = all cores (median)")
+    ap.add_argument("--cpu-turns", type=int, default=2, help="turns of each such run")
+    ap.add_argument("--cpu-samples-p1", type=int, default=3,
+                    help="fresh-copy runs at P = 1 (median); 0 skips them")
+    ap.add_argument("--cpu-turns-p1", type=int, default=1, help="turns of each P = 1 run")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scan-turns", type=int, default=200,
+                    help="turns of the 256-bunch scan (batched vs plain); 0 skips it")
+    ap.add_argument("--scan-batched-only", action="store_true",
+                    help="time the scan's batched path only (no plain runs)")
+    ap.add_argument("--no-main", action="store_true",
+                    help="skip the single-bunch leg (its line then carries no value)")
+    ap.add_argument("--serve-jobs", type=int, default=300,
+                    help="jobs of the serving run (server vs plain FIFO); 0 skips it")
+    ap.add_argument("--serve-rate", type=float, default=150.0, help="job arrivals per second")
+    ap.add_argument("--serve-tick", type=int, default=25, help="turns per server tick")
+    ap.add_argument("--serve-n-max", type=float, default=1e6,
+                    help="largest particle count of a served job (the full stream: 1e7)")
+    ap.add_argument("--serve-turn-range", default="100,2000",
+                    help="turn counts of served jobs, lo,hi (the full stream: 1000,20000)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the turn loop's state of one bunch "
+                                   "lives in one GPU's HBM (no multi-GPU path)",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    import haissinskijulia_amd as H
+    from haissinskijulia_amd import workloads as W
+
+    wl = W.WORKLOADS[args.workload]
+    dev = H.MI355X(0)
+    log(f"device {dev.info()}; workload {args.workload}: n={wl['n']} nbins={wl['nbins']}")
+    out = {"metric": METRIC, "unit": "particle-turns/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic"}
+    cpu_bunch = None
+    if not args.no_main:
+        cpu_bunch = bench_main(H, W, dev, args, wl, out)
     if args.scan_turns > 0:
         log(f"scan: 256 bunches x {args.scan_turns} turns, plain then batched ...")
-        out["scan"] = bench_scan(dev, args.scan_turns)
-        log(f"scan: batched {out['scan']['batched_s']:.2f} s, plain {out['scan']['plain_s']:.2f} s "
-            f"-> {out['scan']['speedup']:.1f}x, identical={out['scan']['results_identical_to_plain']}")
+        out["scan"] = bench_scan(dev, args.scan_turns, not args.scan_batched_only)
+        sc = out["scan"]
+        if "plain_s" in sc:
+            log(f"scan: batched {sc['batched_s']:.2f} s, plain {sc['plain_s']:.2f} s "
+                f"-> {sc['speedup']:.1f}x, identical={sc['results_identical_to_plain']}")
+        else:
+            log(f"scan: batched {sc['batched_s']:.3f} s")
     if args.serve_jobs > 0:
         log(f"serving: {args.serve_jobs} jobs at {args.serve_rate:g}/s, plain then served ...")
         lo, hi = (int(float(x)) for x in args.serve_turn_range.split(","))
@@ -379,7 +391,7 @@ def main() -> int:
         sv = out["serving"]
         log(f"serving: p50 {sv['p50_s']:.3f} s p99 {sv['p99_s']:.3f} s; plain p50 "
             f"{sv['plain_p50_s']:.3f} s p99 {sv['plain_p99_s']:.3f} s")
-    if not args.no_cpu_baseline:
+    if cpu_bunch is not None:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(wl, *cpu_bunch, args.cpu_samples, args.cpu_turns,
                                            args.cpu_samples_p1, args.cpu_turns_p1)

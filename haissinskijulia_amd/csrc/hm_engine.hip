@@ -32,6 +32,22 @@ hm_status fail(hm_status st, const char *fmt, ...) {
     return st;
 }
 
+Arena *&current_arena() {
+    static thread_local Arena *a = nullptr;
+    return a;
+}
+
+void *arena_take(Arena *a, size_t bytes) {
+    const size_t off = (a->used + 255) & ~(size_t)255;
+    if (off > a->cap || bytes > a->cap - off) {
+        fail(HM_ERR_LIMIT, "the allocation-free scratch (%zu bytes) is too small: %zu more bytes "
+                           "needed at offset %zu", a->cap, bytes, off);
+        return nullptr;
+    }
+    a->used = off + bytes;
+    return a->base + off;
+}
+
 hm_status set_device(hm_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
     return HM_OK;
@@ -145,19 +161,31 @@ int auto_fft_block(int64_t nw, int64_t nbins, int max_block) {
 }
 
 hm_status twiddles(hm_ctx *ctx, int F, const double **out) {
-    auto &buf = ctx->twiddles[F];
-    if (!buf.p) {
-        HM_TRY(buf.ensure((size_t)F));  // F/2 complex
-        std::vector<double> h((size_t)F);
-        const long double tp = 6.283185307179586476925286766559L;
-        for (int k = 0; k < F / 2; ++k) {
-            const long double a = tp * (long double)k / (long double)F;
-            h[2 * k] = (double)cosl(a);
-            h[2 * k + 1] = -(double)sinl(a);
-        }
-        HIP_TRY(hipMemcpy(buf.p, h.data(), sizeof(double) * (size_t)F, hipMemcpyHostToDevice));
+    auto it = ctx->twiddles.find(F);
+    if (it != ctx->twiddles.end() && it->second.p) {
+        *out = it->second.p;
+        return HM_OK;
     }
-    *out = buf.p;
+    // an allocation-free call takes a table it has not cached from its scratch
+    // (valid for that call), any other call caches one per context
+    double *p = nullptr;
+    if (Arena *a = current_arena()) {
+        p = static_cast<double *>(arena_take(a, sizeof(double) * (size_t)F));
+        if (!p) return HM_ERR_LIMIT;
+    } else {
+        auto &buf = ctx->twiddles[F];
+        HM_TRY(buf.ensure((size_t)F));  // F/2 complex
+        p = buf.p;
+    }
+    std::vector<double> h((size_t)F);
+    const long double tp = 6.283185307179586476925286766559L;
+    for (int k = 0; k < F / 2; ++k) {
+        const long double a = tp * (long double)k / (long double)F;
+        h[2 * k] = (double)cosl(a);
+        h[2 * k + 1] = -(double)sinl(a);
+    }
+    HIP_TRY(hipMemcpy(p, h.data(), sizeof(double) * (size_t)F, hipMemcpyHostToDevice));
+    *out = p;
     return HM_OK;
 }
 
@@ -405,7 +433,7 @@ hm_status Batch::epilogue(const std::vector<int32_t> &which) {
         // compute_std spreads of the serial branch (evolution.jl:464-467)
         if (q.o.fom_stats || q.o.serial) {
             HM_TRY(q.b->w.stats.ensure((size_t)3 * kStatBlocks));
-            HIP_TRY(hipMemsetAsync(q.d.out + 9, 0, 5 * sizeof(double), s));
+            HIP_TRY(launch_zero(q.d.out + 9, 5 * sizeof(double), s));
             HIP_TRY(launch_pair_stats(q.b->z, q.b->dE, q.b->n, q.d.out + 9, q.b->w.stats.p, s));
         }
         q.active = false;
@@ -500,11 +528,11 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         HM_TRY(w.moments.ensure((size_t)std::max(d.ntiles, 1) * 4));
         HM_TRY(w.sc.ensure(1));
         HM_TRY(w.out.ensure(16));
-        HIP_TRY(hipMemsetAsync(w.out.p, 0, 16 * sizeof(double), s));
+        HIP_TRY(launch_zero(w.out.p, 16 * sizeof(double), s));
         d.nred = red_slices(d.ntiles);
         HM_TRY(w.red.ensure((size_t)d.nred * 4));
         HM_TRY(w.cnt.ensure(1));
-        HIP_TRY(hipMemsetAsync(w.cnt.p, 0, sizeof(uint32_t), s));   // set-up, never in a graph
+        HIP_TRY(launch_zero(w.cnt.p, sizeof(uint32_t), s));
         d.red = w.red.p;
         d.cnt = w.cnt.p;
         d.part = w.part.p;
@@ -521,7 +549,7 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         }
         if (q.o.monitor && q.n_turns > 0) {   // copied to q.o.monitor by run_batch
             HM_TRY(w.mon.ensure((size_t)q.n_turns * 4));
-            HIP_TRY(hipMemsetAsync(w.mon.p, 0, (size_t)q.n_turns * 4 * sizeof(double), s));
+            HIP_TRY(launch_zero(w.mon.p, (size_t)q.n_turns * 4 * sizeof(double), s));
             d.mon = w.mon.p;
         }
         q.o.normals = nullptr;  // host memory of the caller: not kept
@@ -657,8 +685,8 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
             HIP_TRY(launch_resonator_table(w.Wtmp.p, (int)nb, step, rset, s));
         else
             HIP_TRY(hipMemcpyAsync(w.Wtmp.p, W.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemsetAsync(w.P.p, 0, (size_t)nb * 8, s));
-        HIP_TRY(hipMemsetAsync(w.P2.p, 0, (size_t)nb * 16, s));
+        HIP_TRY(launch_zero(w.P.p, (size_t)nb * 8, s));
+        HIP_TRY(launch_zero(w.P2.p, (size_t)nb * 16, s));
         d.P2 = w.P2.p;
         keep.push_back(std::move(W));
         keep.push_back(std::move(lam));
@@ -807,7 +835,7 @@ hm_status Batch::advance(int64_t turns) {
         for (size_t i = 0; i < rb_.size(); ++i)
             if (used_[i] && rb_[i].active) active_particles += rb_[i].d.n;
         int64_t did = 1;
-        if (!timing_ && left >= kGraphTurns && active_particles < kGraphMaxParticles) {
+        if (graphs_ && !timing_ && left >= kGraphTurns && active_particles < kGraphMaxParticles) {
             if (!graph_exec_) HM_TRY(capture_graph());
             HIP_TRY(hipGraphLaunch(graph_exec_, ctx_->stream));
             did = kGraphTurns;

@@ -33,14 +33,41 @@ hm_status fail(hm_status st, const char *fmt, ...);
         if (s_ != HM_OK) return s_; \
     } while (0)
 
+// A region of device memory handed out in order, 256-byte aligned, for the
+// work buffers of one allocation-free call (hm_evolve_device_noalloc): while
+// an ArenaScope is active on a thread, DevBuf::ensure carves from it instead
+// of calling hipMalloc, and release() gives nothing back.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+Arena *&current_arena();
+struct ArenaScope {
+    Arena *prev;
+    explicit ArenaScope(Arena *a) : prev(current_arena()) { current_arena() = a; }
+    ~ArenaScope() { current_arena() = prev; }
+    ArenaScope(const ArenaScope &) = delete;
+    ArenaScope &operator=(const ArenaScope &) = delete;
+};
+// n bytes of the current arena, or null (with the error set) when it is full
+void *arena_take(Arena *a, size_t bytes);
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t cap = 0;  // elements
+    bool from_arena = false;
     hm_status ensure(size_t n) {
         if (n <= cap && p) return HM_OK;
         release();
         const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        if (Arena *a = current_arena()) {
+            p = static_cast<T *>(arena_take(a, bytes));
+            if (!p) return HM_ERR_LIMIT;
+            from_arena = true;
+            cap = std::max<size_t>(n, 1);
+            return HM_OK;
+        }
         hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             p = nullptr;
@@ -50,9 +77,10 @@ struct DevBuf {
         return HM_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !from_arena) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        from_arena = false;
     }
     size_t bytes() const { return cap * sizeof(T); }
 };
@@ -92,6 +120,7 @@ struct hm_ctx {
     hm::DevBuf<uint32_t> shist;
     hm::DevBuf<hm::TurnScalars> ssc;
     hm::DevBuf<hm::BunchDev> sbd;
+    void *scratch = nullptr;      // this device's allocation-free scratch (hm_noalloc.hip)
     std::recursive_mutex mu;
 };
 
@@ -105,6 +134,10 @@ struct hm_bunch {
 namespace hm {
 
 hm_status set_device(hm_ctx *ctx);
+// the device address of the allocation-free scratch on ctx's device (the
+// device must be current), and its size
+hm_status noalloc_scratch(void **ptr);
+size_t noalloc_scratch_bytes();
 double rn_ratio(double x, int64_t num, int64_t den);
 void bin_grid(double zw, int64_t nb, double *edges, double *centers, double *step);
 double particle_factor(int64_t N);
@@ -152,6 +185,7 @@ class Batch {
     size_t size() const { return rb_.size(); }
     RunBunch &slot(int i) { return rb_[(size_t)i]; }
     void set_timing(bool t) { timing_ = t; }
+    void set_graphs(bool g) { graphs_ = g; }   // false: every turn launched directly
     double ms_pass() const { return pass_count_ ? pass_ms_ / (double)pass_count_ : 0.0; }
     double ms_conv() const { return conv_count_ ? conv_ms_ / (double)conv_count_ : 0.0; }
     int64_t pass_count() const { return pass_count_; }
@@ -201,6 +235,7 @@ class Batch {
     size_t interp_lds_ = 0;                         // dynamic LDS of the interpolation sums
     bool dirty_ = true;
     bool timing_ = false;
+    bool graphs_ = true;
     std::vector<hipEvent_t> ev_pool_;
     size_t ev_used_ = 0;
     double pass_ms_ = 0.0, conv_ms_ = 0.0;

@@ -2043,6 +2043,21 @@ hipError_t launch_finalize_whole(const BunchDev *bds, const int32_t *list, int n
     return hipGetLastError();
 }
 
+__global__ void k_zero(uint32_t *__restrict__ p, int64_t words) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s) {
+    const int64_t words = (int64_t)(bytes / 4);
+    if (words <= 0) return hipSuccess;
+    const int64_t b = (words + 255) / 256;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)(b < 1024 ? b : 1024)), dim3(256), 0, s,
+                       static_cast<uint32_t *>(p), words);
+    return hipGetLastError();
+}
+
 hipError_t launch_resonator_table(double *W, int nb, double step, const ResonatorSet &rs,
                                   hipStream_t s) {
     hipLaunchKernelGGL(k_resonator_table, dim3(blocks_for(nb, 256)), dim3(256), 0, s, W, nb, step,

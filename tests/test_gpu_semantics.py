@@ -7,7 +7,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import COORD_RTOL, W, normwise, to_sim_params
+from helpers import COORD_RTOL, POTENTIAL_RTOL, W, normwise, to_sim_params
 from oracle import oracle as O
 
 import haissinskijulia_amd as H
@@ -29,7 +29,7 @@ def test_serial_branch_matches_oracle(flags):
     """evolution.jl:245-253,437-447,461-467: compute_std (two passes) for
     sigma_E0 (the QE amplitude) and the final spreads.  The bunch's energy
     mean is 100 sigma_E off zero, where the MPI branch's one-pass spread
-    differs from the two-pass one by ~2e-11."""
+    loses digits against the two-pass one (~1e-12 relative here)."""
     p = W.ring_3gev_lattice(40, flags=flags)
     n = 20_001
     z0, e0 = O.generate_gaussian(31, 2e-4, 4e-3, 3e8, 3e6, n)
@@ -45,9 +45,10 @@ def test_serial_branch_matches_oracle(flags):
     assert sz == pytest.approx(rr.sigma_z, rel=1e-13)
     # the GPU's initial spread is the two-pass one too (it sets the excitation)
     assert H.longitudinal_evolve.last.sigma_E0 == pytest.approx(O.serial_std(e0), rel=1e-14)
-    # and the MPI branch's spreads are measurably different on this bunch
+    # and the MPI branch's one-pass spread is measurably further off: by more
+    # than ten times the serial branch's distance to the oracle
     z1, e1, (sE1, _, _) = _evolve(z0, e0, 0, p, use_mpi=True, normals=nrm)
-    assert abs(sE1 - sE) > 1e-12 * sE
+    assert abs(sE1 - sE) > 10 * abs(sE - rr.sigma_E) and abs(sE1 - sE) > 1e-14 * sE
 
 
 def test_serial_branch_edge_cases():
@@ -135,7 +136,8 @@ def test_lag_wake_potential_is_causal_on_the_gpu():
     zr, er, rr = O.evolve(z0, e0, 512, p, seed=1, **kw)
     assert rr.status == 0
     z, e, _ = _evolve(z0, e0, 512, p, seed=1, **kw)
-    assert normwise(e, er) <= COORD_RTOL
+    # dE is the kick alone here, so the potential's tolerance applies
+    assert normwise(e, er) <= POTENTIAL_RTOL
     kick = -e[60_000:]
     ahead = probes > 4e-3 + 6 * 5e-4
     behind = (probes > -1e-3) & (probes < 1e-3)
