@@ -270,12 +270,10 @@ def bench_serve(dev, count: int, rate: float, turns_per_tick: int, n_max: float 
 def bench_main(H, W, dev, args, wl: dict, out: dict):
     """The single-bunch leg: warm-up, the timed evolve call (the JSON line's
     value), a separately timed run for the pass kernel's roofline, and the
-    plain-copy ceiling.  Returns the initial bunch for the CPU baseline."""
+    plain-copy ceiling."""
     n, nbins = wl["n"], wl["nbins"]
     bunch = H.DeviceBunch(n, dev)
     bunch.generate(wl["seed"], W.SIGMA_Z, W.SIGMA_E)
-    # the initial bunch for the CPU baseline: the same particles
-    cpu_bunch = None if args.no_cpu_baseline else bunch.download()
 
     def params(turns):
         return H.SimulationParameters.from_dict(W.ring_3gev_lattice(n_turns=turns))
@@ -321,7 +319,16 @@ def bench_main(H, W, dev, args, wl: dict, out: dict):
         "stream_copy_GBps": copy_gbs,
         "conv_ms": rt.ms_conv_avg,
     })
-    return cpu_bunch
+
+
+def initial_bunch(H, W, dev, wl: dict):
+    """The workload's initial bunch on the host (the particles the GPU legs
+    start from), for the CPU baseline; fetched after the GPU legs."""
+    b = H.DeviceBunch(wl["n"], dev)
+    b.generate(wl["seed"], W.SIGMA_Z, W.SIGMA_E)
+    z, e = b.download()
+    b.close()
+    return z, e
 
 
 def main() -> int:
@@ -371,9 +378,8 @@ def main() -> int:
     out = {"metric": METRIC, "unit": "particle-turns/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic"}
-    cpu_bunch = None
     if not args.no_main:
-        cpu_bunch = bench_main(H, W, dev, args, wl, out)
+        bench_main(H, W, dev, args, wl, out)
     if args.scan_turns > 0:
         log(f"scan: 256 bunches x {args.scan_turns} turns, plain then batched ...")
         out["scan"] = bench_scan(dev, args.scan_turns, not args.scan_batched_only)
@@ -391,10 +397,10 @@ def main() -> int:
         sv = out["serving"]
         log(f"serving: p50 {sv['p50_s']:.3f} s p99 {sv['p99_s']:.3f} s; plain p50 "
             f"{sv['plain_p50_s']:.3f} s p99 {sv['plain_p99_s']:.3f} s")
-    if cpu_bunch is not None:
+    if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(wl, *cpu_bunch, args.cpu_samples, args.cpu_turns,
-                                           args.cpu_samples_p1, args.cpu_turns_p1)
+        out["cpu_baseline"] = cpu_baseline(wl, *initial_bunch(H, W, dev, wl), args.cpu_samples,
+                                           args.cpu_turns, args.cpu_samples_p1, args.cpu_turns_p1)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} particle-turns/s")
     else:
         out["cpu_baseline"] = None

@@ -48,6 +48,52 @@ void *arena_take(Arena *a, size_t bytes) {
     return a->base + off;
 }
 
+hm_status stage_begin(hm_ctx *ctx, size_t bytes) {
+    Staging &g = ctx->stage;
+    g.used = 0;
+    if (current_arena()) return HM_OK;
+    if (g.pending) {
+        HIP_TRY(hipEventSynchronize(g.done));
+        g.pending = false;
+    }
+    if (bytes > g.cap) {
+        if (g.host) (void)hipHostFree(g.host);
+        g.host = nullptr;
+        g.cap = 0;
+        const size_t c = std::max<size_t>(stage_size(bytes), size_t(1) << 20);
+        HIP_TRY(hipHostMalloc((void **)&g.host, c, hipHostMallocDefault));
+        g.cap = c;
+    }
+    if (!g.done) HIP_TRY(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    return HM_OK;
+}
+
+hm_status stage_copy(hm_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return HM_OK;
+    Staging &g = ctx->stage;
+    if (current_arena() || !g.host) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return HM_OK;
+    }
+    const size_t off = stage_size(g.used);
+    if (off + bytes > g.cap) return fail(HM_ERR_LIMIT, "staging round overflows its size");
+    std::memcpy(g.host + off, src, bytes);
+    g.used = off + bytes;
+    HIP_TRY(hipMemcpyAsync(dst, g.host + off, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return HM_OK;
+}
+
+hm_status stage_end(hm_ctx *ctx) {
+    Staging &g = ctx->stage;
+    if (current_arena() || !g.host) {   // pageable sources: done before they go
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return HM_OK;
+    }
+    HIP_TRY(hipEventRecord(g.done, ctx->stream));
+    g.pending = true;
+    return HM_OK;
+}
+
 hm_status set_device(hm_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
     return HM_OK;
@@ -273,14 +319,17 @@ int Batch::active_count() const {
 }
 
 hm_status Batch::upload_descriptors() {
+    HM_TRY(stage_begin(ctx_, stage_size(rb_.size() * sizeof(BunchDev))));
+    HM_TRY(stage_descriptors());
+    return stage_end(ctx_);
+}
+
+// the descriptors' copy within a staging round
+hm_status Batch::stage_descriptors() {
     HM_TRY(d_bd_.ensure(std::max<size_t>(rb_.size(), 1)));
     std::vector<BunchDev> h(rb_.size());
     for (size_t i = 0; i < rb_.size(); ++i) h[i] = rb_[i].d;
-    if (!h.empty())
-        HIP_TRY(hipMemcpyAsync(d_bd_.p, h.data(), h.size() * sizeof(BunchDev), hipMemcpyHostToDevice,
-                               ctx_->stream));
-    HIP_TRY(hipStreamSynchronize(ctx_->stream));  // h goes out of scope
-    return HM_OK;
+    return stage_copy(ctx_, d_bd_.p, h.data(), h.size() * sizeof(BunchDev));
 }
 
 // launch tables of the active bunches
@@ -371,28 +420,28 @@ hm_status Batch::build_tables() {
     std::vector<int32_t> ranges;
     balanced_ranges(wpass, pass_grid_, ranges);
     balanced_ranges(winterp, interp_grid_, ranges);
-    hipStream_t s = ctx_->stream;
+    // one staging round for every table and the descriptors
+    size_t bytes = stage_size(ranges.size() * 4) + stage_size(list.size() * 4) +
+                   stage_size(red.size() * 4) + stage_size(tiles.size() * 4) +
+                   stage_size(rb_.size() * sizeof(BunchDev));
+    for (int k = 0; k < kConvKinds; ++k) bytes += stage_size(conv[k].size() * 4);
     HM_TRY(d_ranges_.ensure(std::max<size_t>(ranges.size(), 1)));
-    if (!ranges.empty())
-        HIP_TRY(hipMemcpyAsync(d_ranges_.p, ranges.data(), ranges.size() * 4, hipMemcpyHostToDevice,
-                               s));
     HM_TRY(d_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
     HM_TRY(d_red_.ensure(std::max<size_t>(red.size(), 2)));
     HM_TRY(d_list_.ensure(std::max<size_t>(list.size(), 1)));
-    if (!list.empty())
-        HIP_TRY(hipMemcpyAsync(d_list_.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
-    if (!red.empty())
-        HIP_TRY(hipMemcpyAsync(d_red_.p, red.data(), red.size() * 4, hipMemcpyHostToDevice, s));
     for (int k = 0; k < kConvKinds; ++k) {
         nconv_[k] = (int)conv[k].size() / 2;
         HM_TRY(d_conv_[k].ensure(std::max<size_t>(conv[k].size(), 2)));
-        if (!conv[k].empty())
-            HIP_TRY(hipMemcpyAsync(d_conv_[k].p, conv[k].data(), conv[k].size() * 4,
-                                   hipMemcpyHostToDevice, s));
     }
-    if (!tiles.empty())
-        HIP_TRY(hipMemcpyAsync(d_tiles_.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, s));
-    HM_TRY(upload_descriptors());  // synchronises: the vectors above are still alive
+    HM_TRY(stage_begin(ctx_, bytes));
+    HM_TRY(stage_copy(ctx_, d_ranges_.p, ranges.data(), ranges.size() * 4));
+    HM_TRY(stage_copy(ctx_, d_list_.p, list.data(), list.size() * 4));
+    HM_TRY(stage_copy(ctx_, d_red_.p, red.data(), red.size() * 4));
+    for (int k = 0; k < kConvKinds; ++k)
+        HM_TRY(stage_copy(ctx_, d_conv_[k].p, conv[k].data(), conv[k].size() * 4));
+    HM_TRY(stage_copy(ctx_, d_tiles_.p, tiles.data(), tiles.size() * 4));
+    HM_TRY(stage_descriptors());
+    HM_TRY(stage_end(ctx_));
     dirty_ = false;
     return HM_OK;
 }
@@ -410,13 +459,14 @@ hm_status Batch::run_subset_moments(const std::vector<int32_t> &which, int mode)
     }
     HM_TRY(d_tmp_tiles_.ensure(std::max<size_t>(tiles.size(), 1)));
     HM_TRY(d_tmp_list_.ensure(which.size()));
-    if (!tiles.empty())
-        HIP_TRY(hipMemcpyAsync(d_tmp_tiles_.p, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(d_tmp_list_.p, which.data(), which.size() * 4, hipMemcpyHostToDevice, s));
-    HM_TRY(upload_descriptors());
+    HM_TRY(stage_begin(ctx_, stage_size(tiles.size() * 4) + stage_size(which.size() * 4) +
+                                 stage_size(rb_.size() * sizeof(BunchDev))));
+    HM_TRY(stage_copy(ctx_, d_tmp_tiles_.p, tiles.data(), tiles.size() * 4));
+    HM_TRY(stage_copy(ctx_, d_tmp_list_.p, which.data(), which.size() * 4));
+    HM_TRY(stage_descriptors());
+    HM_TRY(stage_end(ctx_));
     HIP_TRY(launch_moments(d_bd_.p, d_tmp_tiles_.p, (int)tiles.size(), mode, s));
     HIP_TRY(launch_moments_final(d_bd_.p, d_tmp_list_.p, (int)which.size(), mode, s));
-    HIP_TRY(hipStreamSynchronize(s));
     for (size_t k = 0; k < which.size(); ++k) rb_[(size_t)which[k]].d.tile_begin = saved[k];
     dirty_ = true;  // descriptors on the device carry the temporary tile_begin
     return HM_OK;
@@ -566,7 +616,9 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
                 q.res_copy.assign(spec.o.resonators, spec.o.resonators + 3 * spec.o.n_resonators);
         }
     }
-    HIP_TRY(hipStreamSynchronize(s));  // normals copies read the caller's memory
+    bool injected = false;
+    for (int32_t i : fresh) injected = injected || (rb_[(size_t)i].d.flags & F_INJ);
+    if (injected) HIP_TRY(hipStreamSynchronize(s));  // the normals' copies read the caller's memory
     // initial spreads (evolution.jl:245-253): compute_global_std, or for the
     // serial branch compute_std (two passes) into out[0..1]
     HM_TRY(run_subset_moments(fresh, 0));
@@ -579,12 +631,14 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         HIP_TRY(hipMemcpyAsync(q.d.out + 1, q.d.out + 11, sizeof(double), hipMemcpyDeviceToDevice,
                                s));
     }
-    std::vector<std::vector<double>> keep;   // host tables alive until the copies finish
+    // host tables, uploaded in one staging round after the loop
+    std::vector<std::pair<double *, std::vector<double>>> keep;
     for (int32_t i : fresh) {
         RunBunch &q = rb_[(size_t)i];
         BunchDev &d = q.d;
         double o2[2];
-        HIP_TRY(hipMemcpy(o2, d.out, sizeof(o2), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(o2, d.out, sizeof(o2), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
         d.sigma_E0 = o2[0];
         const double sz0 = o2[1];
         if (!(d.flags & F_WAKE)) continue;
@@ -615,12 +669,14 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         d.e_last = zw;
         d.inv_step = 1.0 / step;                                              // utils.jl:378
         d.inv_s = nb > 1 ? 1.0 / ((d.c_last - d.c_first) / (double)(nb - 1)) : 0.0;  // wakefield.jl:39-42
-        std::vector<double> W, lam((size_t)nb);
+        std::vector<double> W, lam;
         const double wf = ((120.0 * kPi) * kSpeedLight) / (kPi * q.p.pipe_radius);   // evolution.jl:279
         const double ws = std::sqrt(jl_max_h(0.0, (2.0 * 30.0) / q.p.pipe_radius));  // evolution.jl:280
         const double inv_ctau = 1.0 / 4e-3;                                          // wakefield.jl:158
-        for (int64_t k = 0; k < nb; ++k)
-            lam[(size_t)k] = dsig > 0 ? ref_delta(centers[(size_t)k], dsig) : 1.0;
+        if (dsig > 0) {   // else lam = 1, filled on the device
+            lam.resize((size_t)nb);
+            for (int64_t k = 0; k < nb; ++k) lam[(size_t)k] = ref_delta(centers[(size_t)k], dsig);
+        }
         // the wake table: the caller's, the reference's W(c_j) on the host, or
         // the resonators' W(j step) computed on the device (k_resonator_table;
         // a resonator's wake has no zero tail on any grid here: support nb)
@@ -680,16 +736,17 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
                 d.Y = w.Y.p;
             }
         }
-        HIP_TRY(hipMemcpyAsync(w.lam.p, lam.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
+        if (lam.empty())
+            HIP_TRY(launch_fill(w.lam.p, (size_t)nb, 1.0, s));
+        else
+            keep.emplace_back(w.lam.p, std::move(lam));
         if (rset.n > 0)
             HIP_TRY(launch_resonator_table(w.Wtmp.p, (int)nb, step, rset, s));
         else
-            HIP_TRY(hipMemcpyAsync(w.Wtmp.p, W.data(), (size_t)nb * 8, hipMemcpyHostToDevice, s));
+            keep.emplace_back(w.Wtmp.p, std::move(W));
         HIP_TRY(launch_zero(w.P.p, (size_t)nb * 8, s));
         HIP_TRY(launch_zero(w.P2.p, (size_t)nb * 16, s));
         d.P2 = w.P2.p;
-        keep.push_back(std::move(W));
-        keep.push_back(std::move(lam));
         d.hist = w.hist.p;
         d.P = w.P.p;
         d.lam = w.lam.p;
@@ -698,17 +755,23 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         HM_TRY(twiddles(ctx_, d.F, &tw));
         d.tw = tw;
     }
-    HM_TRY(upload_descriptors());   // also waits for the table copies above
-    keep.clear();
+    {   // the host tables, the descriptors and the new bunches' list: one round
+        HM_TRY(d_tmp_list_.ensure(fresh.size()));
+        size_t bytes = stage_size(rb_.size() * sizeof(BunchDev)) + stage_size(fresh.size() * 4);
+        for (auto &kv : keep) bytes += stage_size(kv.second.size() * 8);
+        HM_TRY(stage_begin(ctx_, bytes));
+        for (auto &kv : keep) HM_TRY(stage_copy(ctx_, kv.first, kv.second.data(), kv.second.size() * 8));
+        HM_TRY(stage_descriptors());
+        HM_TRY(stage_copy(ctx_, d_tmp_list_.p, fresh.data(), fresh.size() * 4));
+        HM_TRY(stage_end(ctx_));
+        keep.clear();
+    }
     for (int32_t i : fresh) {
         RunBunch &q = rb_[(size_t)i];
         if (q.d.flags & F_WAKE)
             HIP_TRY(launch_fft_wake(d_bd_.p + i, q.b->w.Wtmp.p, q.d.nJ, q.d.F, s));
     }
-    HM_TRY(d_tmp_list_.ensure(fresh.size()));
-    HIP_TRY(hipMemcpyAsync(d_tmp_list_.p, fresh.data(), fresh.size() * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_init(d_bd_.p, d_tmp_list_.p, (int)fresh.size(), s));
-    HIP_TRY(hipStreamSynchronize(s));
     std::vector<int32_t> zero_turn;
     for (int32_t i : fresh)
         if (rb_[(size_t)i].n_turns == 0) zero_turn.push_back(i);
@@ -862,7 +925,8 @@ hm_status Batch::result(int i, hm_result *r, double *cz, double *cE) {
     if (i < 0 || (size_t)i >= rb_.size() || !used_[(size_t)i]) return fail(HM_ERR_ARG, "bad slot");
     RunBunch &q = rb_[(size_t)i];
     double o[16];
-    HIP_TRY(hipMemcpy(o, q.d.out, sizeof(o), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(o, q.d.out, sizeof(o), hipMemcpyDeviceToHost, ctx_->stream));
+    HIP_TRY(hipStreamSynchronize(ctx_->stream));
     std::memset(r, 0, sizeof(*r));
     r->sigma_E0 = o[0];
     r->sigma_z0 = o[1];
@@ -956,9 +1020,11 @@ static hm_status run_batch(hm_ctx *ctx, int32_t nb, hm_bunch *const *bunches, co
         results[orig[k]].ms_total = total_ms;
         const RunBunch &q = specs[k];
         if (q.o.monitor && q.p.n_turns > 0)
-            HIP_TRY(hipMemcpy(q.o.monitor, q.b->w.mon.p, (size_t)q.p.n_turns * 4 * sizeof(double),
-                              hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpyAsync(q.o.monitor, q.b->w.mon.p,
+                                   (size_t)q.p.n_turns * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
     }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return HM_OK;
 }
 
@@ -1002,6 +1068,8 @@ hm_status hm_destroy(hm_ctx *ctx) {
     ctx->shist.release();
     ctx->ssc.release();
     ctx->sbd.release();
+    if (ctx->stage.host) (void)hipHostFree(ctx->stage.host);
+    if (ctx->stage.done) (void)hipEventDestroy(ctx->stage.done);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return HM_OK;

@@ -106,6 +106,18 @@ struct Work {
     }
 };
 
+// Uploads of small host tables through one pinned staging buffer per
+// context: a round packs its pieces into the buffer, enqueues their copies
+// without waiting and records an event; the next round waits for that event
+// before it packs again.  In an allocation-free call the copies come from the
+// caller's pageable memory and the round ends with a stream synchronisation.
+struct Staging {
+    char *host = nullptr;
+    size_t cap = 0, used = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+
 }  // namespace hm
 
 struct hm_ctx {
@@ -121,6 +133,7 @@ struct hm_ctx {
     hm::DevBuf<hm::TurnScalars> ssc;
     hm::DevBuf<hm::BunchDev> sbd;
     void *scratch = nullptr;      // this device's allocation-free scratch (hm_noalloc.hip)
+    hm::Staging stage;            // pinned staging of table uploads
     std::recursive_mutex mu;
 };
 
@@ -134,6 +147,12 @@ struct hm_bunch {
 namespace hm {
 
 hm_status set_device(hm_ctx *ctx);
+// a staging round of at most `bytes` (each piece aligned to 64), its copies
+// on ctx's stream, and its end
+hm_status stage_begin(hm_ctx *ctx, size_t bytes);
+hm_status stage_copy(hm_ctx *ctx, void *dst, const void *src, size_t bytes);
+hm_status stage_end(hm_ctx *ctx);
+inline size_t stage_size(size_t bytes) { return (bytes + 63) & ~(size_t)63; }
 // the device address of the allocation-free scratch on ctx's device (the
 // device must be current), and its size
 hm_status noalloc_scratch(void **ptr);
@@ -194,6 +213,7 @@ class Batch {
     hm_status add_slots(const std::vector<RunBunch> &specs, std::vector<int32_t> &fresh);
     hm_status build_tables();
     hm_status upload_descriptors();
+    hm_status stage_descriptors();
     hm_status run_subset_moments(const std::vector<int32_t> &which, int mode);
     hm_status epilogue(const std::vector<int32_t> &which);
     hm_status one_turn();

@@ -187,8 +187,12 @@ __device__ __forceinline__ void interp_index_pass(double z, double c0, double c1
 }
 __device__ __forceinline__ void interp_index(double z, double c0, double c1, double inv, int nb,
                                              int &k, double &w) {
-    interp_index_pass(z, c0, c1, inv, nb, k, w);
-    if (z != z) w = z;
+    const double cl = __builtin_fmin(__builtin_fmax(z, c0), c1);
+    const double u = (cl - c0) * inv;
+    const double fu = floor(u);
+    const int kk = (fu == fu) ? (int)fu : 0;
+    k = min(max(kk, 0), nb - 2);
+    w = (z == z) ? u - fu : z;
 }
 
 // the interpolated value (wakefield.jl:94-102)
@@ -1536,6 +1540,12 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_sum_deep(
     interp_sum_blocks<8>(bds, tile_bunch, ranges);
 }
 
+// Per-tile sums of z, z^2, dE, dE^2 (the initial and final spreads,
+// evolution.jl:245-253,461-467); mode 1 first applies the last turn's pending
+// wake kick and re-centre (the epilogue) and writes dE back.  The tile is read
+// in the turn pass's layout, 16-byte (z, z) and (dE, dE) pairs, thread t taking
+// pairs lt*ppt + t + k*1024, two pairs' loads in flight; the single particle
+// of an odd bunch goes to thread 0 of the bunch's last tile.
 __global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__restrict__ bds,
                                                           const int32_t *__restrict__ tile_bunch,
                                                           int mode) {
@@ -1543,23 +1553,49 @@ __global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__rest
     const int tile = blockIdx.x;
     const BunchDev &B = bds[tile_bunch[tile]];
     const int lt = tile - B.tile_begin;
-    const int64_t i0 = 2 * (int64_t)lt * B.pairs_per_tile;
-    const int64_t i1 = min(i0 + 2 * B.pairs_per_tile, B.n);
+    const int64_t ppt = B.pairs_per_tile;
+    const int64_t p0 = (int64_t)lt * ppt;
+    const int64_t p1 = min(p0 + ppt, B.npairs);
+    const int64_t full = B.n >> 1;                  // pairs with two particles
+    const int64_t pf = min(p1, full);
     PassCtx c;
     if (mode == 1) load_ctx(B, c);
-    const bool pend = (mode == 1) && (c.pending_kick || c.pending_m);
+    const bool kick = (mode == 1) && c.pending_kick;
+    const bool rec = (mode == 1) && c.pending_m;
+    g_f64 *zp = (g_f64 *)B.z;
+    g_f64 *ep = (g_f64 *)B.dE;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const double z = B.z[i];
-        double e = B.dE[i];
-        if (pend) {
-            e = apply_pending(c, z, e);
-            B.dE[i] = e;
-        }
+    auto one = [&](double z, double e) -> double {
+        if (kick) e = e - interp_pot(c.P, z, c.c0, c.c1, c.inv_s, c.nb);   // wakefield.jl:342-344
+        if (rec) e = e - c.m_prev;                                          // evolution.jl:434
         v[0] += z;
         v[1] += z * z;
         v[2] += e;
         v[3] += e * e;
+        return e;
+    };
+    constexpr int64_t kStep = kPassThreads;
+    for (int64_t p = p0 + threadIdx.x; p < pf; p += 2 * kStep) {
+        const int64_t q = p + kStep;
+        const bool two = q < pf;
+        const double2 za = ldnt((const g_f64x2 *)(zp + 2 * p));
+        const double2 ea = ldnt((const g_f64x2 *)(ep + 2 * p));
+        double2 zb = make_double2(0.0, 0.0), eb = make_double2(0.0, 0.0);
+        if (two) {
+            zb = ldnt((const g_f64x2 *)(zp + 2 * q));
+            eb = ldnt((const g_f64x2 *)(ep + 2 * q));
+        }
+        const double a0 = one(za.x, ea.x), a1 = one(za.y, ea.y);
+        if (kick || rec) stnt((g_f64x2 *)(ep + 2 * p), a0, a1);
+        if (two) {
+            const double b0 = one(zb.x, eb.x), b1 = one(zb.y, eb.y);
+            if (kick || rec) stnt((g_f64x2 *)(ep + 2 * q), b0, b1);
+        }
+    }
+    if ((B.n & 1) && full >= p0 && full < p1 && threadIdx.x == 0) {
+        const int64_t i = B.n - 1;
+        const double e = one(zp[i], ep[i]);
+        if (kick || rec) ep[i] = e;
     }
     block_sum<4>(v, s_red);
     if (threadIdx.x == 0) {
@@ -2047,6 +2083,20 @@ __global__ void k_zero(uint32_t *__restrict__ p, int64_t words) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0u;
+}
+
+__global__ void k_fill(double *__restrict__ p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const int64_t b = ((int64_t)n + 255) / 256;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)(b < 1024 ? b : 1024)), dim3(256), 0, s, p,
+                       (int64_t)n, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s) {

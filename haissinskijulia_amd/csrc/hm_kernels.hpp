@@ -55,6 +55,8 @@ inline bool conv_spec_fits(int nA, int nJ, int F, int lds_max) {
 // zero `bytes` (a multiple of 4) at p by a kernel (no memset node: callers may
 // capture the stream)
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
+// p[0..n) = v by a kernel
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t s);
 // allow the dynamic-LDS kernels the device's LDS per block (best effort)
 hipError_t configure_kernels(int max_dynamic_lds);
 
