@@ -118,6 +118,8 @@ def load() -> C.CDLL:
             "hm_bunch_evolve": (st, [vp, C.c_int64, C.POINTER(HmParams), C.POINTER(HmOptions),
                                      C.POINTER(HmResult)]),
             "hm_bunch_device_ptrs": (st, [vp, C.POINTER(vp), C.POINTER(vp)]),
+            "hm_evolve_device_noalloc": (st, [vp, vp, vp, C.c_int64, C.c_int64, C.POINTER(HmParams),
+                                              C.POINTER(HmOptions), C.POINTER(HmResult)]),
             "hm_batch_evolve": (st, [vp, C.c_int32, C.POINTER(vp), _ip, C.POINTER(HmParams),
                                      C.POINTER(HmOptions), C.POINTER(HmResult)]),
             "hm_rf_kick": (st, [vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int64]),

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhaissinski_mi355x.so")
-SOURCES = ["hm_kernels.hip", "hm_engine.hip", "hm_stages.hip", "hm_server.hip"]
+SOURCES = ["hm_kernels.hip", "hm_engine.hip", "hm_stages.hip", "hm_server.hip", "hm_noalloc.hip"]
 ARCH = "gfx950"
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

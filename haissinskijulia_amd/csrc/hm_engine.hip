@@ -1054,6 +1054,7 @@ hm_status hm_create(int32_t device, hm_ctx **out) {
     (void)configure_kernels(lds);
     (void)hipGetLastError();
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HM_TRY(noalloc_scratch(&ctx->scratch));   // this device's copy, with the device current
     *out = ctx.release();
     return HM_OK;
 }

@@ -149,6 +149,17 @@ hm_status hm_bunch_evolve(hm_bunch *b, int64_t nbins, const hm_params *params,
                           const hm_options *opts, hm_result *result);
 /* device addresses of the bunch's z and dE arrays (for callers that own kernels) */
 hm_status hm_bunch_device_ptrs(hm_bunch *b, double **z_device, double **dE_device);
+/* No counterpart in the reference: hm_bunch_evolve on the caller's device
+ * arrays z_device, dE_device (mutated in place) with no device allocation, for
+ * callers that cannot allocate (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc).  Every work buffer comes from an 8 MiB
+ * per-GPU scratch array held until the process ends: at most 32768 bins
+ * (HM_ERR_LIMIT beyond); no injected normals, turn monitor or timing
+ * (HM_ERR_ARG).  The same kernels as hm_bunch_evolve, so the same results.
+ * Calls that overlap on one GPU race on the scratch array. */
+hm_status hm_evolve_device_noalloc(hm_ctx *ctx, double *z_device, double *dE_device, int64_t n,
+                                   int64_t nbins, const hm_params *params,
+                                   const hm_options *opts, hm_result *result);
 
 /* ---- batched bunches: one segmented launch per turn for all of them ----- */
 /* Each bunch keeps its own parameters, options, bin count and RNG stream;
