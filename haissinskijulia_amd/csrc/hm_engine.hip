@@ -367,7 +367,7 @@ hm_status Batch::build_tables() {
         if (q.d.flags & (F_WAKE | F_E0)) any_reduce_ = true;
         if (q.d.flags & F_WAKE) {
             any_wake_ = true;
-            if (q.d.flags & F_E0) {
+            if ((q.d.flags & F_E0) && !(q.d.flags & F_CIC)) {   // the interpolation sums' bunches
                 any_wake_e0_ = true;
                 interp_lds_ = std::max(interp_lds_, (size_t)q.d.iwin * 8);
             }
@@ -396,7 +396,7 @@ hm_status Batch::build_tables() {
                 spec_lds_ = std::max(spec_lds_, conv_spec_lds(q.d.nA, q.d.nJ, q.d.F));
             }
             Fmax_ = std::max(Fmax_, q.d.F);
-            pass_lds_ = std::max(pass_lds_, pass_hist_bytes(q.d) + (size_t)q.d.win * 8);
+            pass_lds_ = std::max(pass_lds_, (size_t)pass_lds_bytes(q.d));
         }
     }
     ntiles_ = (int)tiles.size();
@@ -411,7 +411,7 @@ hm_status Batch::build_tables() {
         const int64_t lt = t - d.tile_begin;
         const int64_t pairs = std::min(d.pairs_per_tile, d.npairs - lt * d.pairs_per_tile);
         wpass[(size_t)t] = pairs;
-        winterp[(size_t)t] = ((d.flags & F_WAKE) && (d.flags & F_E0)) ? pairs : 0;
+        winterp[(size_t)t] = ((d.flags & F_WAKE) && (d.flags & F_E0) && !(d.flags & F_CIC)) ? pairs : 0;
     }
     const int per_cu = interp_lds_ + 1024 <= (size_t)ctx_->lds_max / 2 ? 2 : 1;
     pass_grid_ = std::min(ntiles_, ctx_->cus);
@@ -714,9 +714,21 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         // the histogram in LDS only with room for a window of at least two
         // entries of P beside it (the all-flags pass relies on that)
         if ((size_t)nb * 4 + 16 > (size_t)ctx_->lds_max - 1024) d.flags |= F_GHIST;
-        {   // the pass's LDS window of P: what the histogram leaves of the LDS
-            const int64_t cap = ((int64_t)ctx_->lds_max - 1024 - (int64_t)pass_hist_bytes(d)) / 8;
+        // With update_E0 the pass deposits the interpolation weights of its
+        // post-drift z (cloud in cell) where its LDS holds the counters beside
+        // the histogram and the whole table of P, instead of the interpolation
+        // sums' second read of z.  The counters hold any bunch of fewer than
+        // 2^32 particles; a single bin has no cells.
+        d.flags &= ~F_CIC;
+        if ((d.flags & F_E0) && !(d.flags & F_GHIST) && nb >= 2 && d.n < ((int64_t)1 << 32) &&
+            (int64_t)pass_hist_bytes(d) + 8 * nb + cic_bytes((int)nb) <=
+                (int64_t)ctx_->lds_max - 1024)
+            d.flags |= F_CIC;
+        {   // the pass's LDS window of P: what the histogram (and deposit) leave of the LDS
+            const int64_t cap = ((int64_t)ctx_->lds_max - 1024 - (int64_t)pass_hist_bytes(d) -
+                                 (int64_t)pass_cic_bytes(d)) / 8;
             d.win = cap >= 2 ? (int32_t)std::min<int64_t>(nb, cap) : 0;
+            d.cic_off = (d.flags & F_CIC) ? pass_hist_bytes(d) + ((d.win * 8 + 15) & ~15) : 0;
             // the interpolation sums' window: the whole table when it fits
             d.iwin = (int32_t)std::min<int64_t>(nb, ((int64_t)ctx_->lds_max - 1024) / 8);
         }
@@ -746,6 +758,13 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
             keep.emplace_back(w.Wtmp.p, std::move(W));
         HIP_TRY(launch_zero(w.P.p, (size_t)nb * 8, s));
         HIP_TRY(launch_zero(w.P2.p, (size_t)nb * 16, s));
+        d.cic = nullptr;
+        if (d.flags & F_CIC) {   // cleared again by each turn's finalize
+            const size_t words = 2 * (size_t)nb + ((size_t)nb + 2) / 2;   // hi, lo, n
+            HM_TRY(w.cic.ensure(words));
+            HIP_TRY(launch_zero(w.cic.p, words * 8, s));
+            d.cic = w.cic.p;
+        }
         d.P2 = w.P2.p;
         d.hist = w.hist.p;
         d.P = w.P.p;

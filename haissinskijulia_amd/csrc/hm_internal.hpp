@@ -92,17 +92,18 @@ struct Work {
     DevBuf<TurnScalars> sc;
     DevBuf<double> red, Y;
     DevBuf<uint32_t> cnt;
+    DevBuf<unsigned long long> cic;
     void release() {
         hist.release(); P.release(); P2.release(); lam.release(); What.release(); X.release();
         Wtmp.release(); part.release(); ipart.release(); moments.release(); normals.release();
         out.release(); stats.release(); mon.release(); sc.release(); red.release(); cnt.release();
-        Y.release();
+        Y.release(); cic.release();
     }
     size_t bytes() const {
         return hist.bytes() + P.bytes() + P2.bytes() + lam.bytes() + What.bytes() + X.bytes() +
                Wtmp.bytes() + part.bytes() + ipart.bytes() + moments.bytes() + normals.bytes() +
                out.bytes() + stats.bytes() + mon.bytes() + sc.bytes() + red.bytes() + cnt.bytes() +
-               Y.bytes();
+               Y.bytes() + cic.bytes();
     }
 };
 

@@ -461,6 +461,7 @@ typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) double g_f64;
 typedef __attribute__((address_space(1))) f64x2 g_f64x2;
 typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((address_space(1))) unsigned long long g_u64;
 
 __device__ __forceinline__ double2 ld2(const g_f64x2 *a) {
     const f64x2 v = *a;
@@ -511,6 +512,31 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
     }
 }
 
+// The cloud-in-cell deposit of one post-drift z (F_CIC) into the LDS counters
+// at byte offset cb: the cell and weight the next pass's kick takes for it
+// (interp_index_pass).  round(w 2^51) is the low 52 bits of the double
+// 2^52 + w 2^51, formed by one fused multiply-add (one rounding; w in [0, 1),
+// so the sum stays below 2^53).  A NaN z or a non-finite weight is counted in
+// n[nb] instead.
+__device__ __forceinline__ void cic_deposit(const PassCtx &c, int cb, double z) {
+    int k;
+    double w;
+    interp_index_pass(z, c.c0, c.c1, c.inv_s, c.nb, k, w);
+    unsigned long long *sH = reinterpret_cast<unsigned long long *>(pass_lds() + cb);
+    unsigned long long *sL = sH + c.nb;
+    uint32_t *sN = reinterpret_cast<uint32_t *>(sL + c.nb);
+    if ((z == z) & (w == w)) {
+        const double y = __builtin_fma(w, kCicScale, 2.0 * kCicScale);
+        const unsigned long long W =
+            (unsigned long long)__double_as_longlong(y) & ((1ull << 52) - 1);   // <= 2^51
+        atomicAdd(sN + k, 1u);
+        atomicAdd(sH + k, W >> kCicLoBits);
+        atomicAdd(sL + k, W & ((1ull << kCicLoBits) - 1));
+    } else {
+        atomicAdd(sN + c.nb, 1u);
+    }
+}
+
 // The fused pass over a run of tiles.  FX = kStdFlags is the common configuration
 // with the flags folded at compile time; FX = 0 handles any other.  Both
 // evaluate the same expressions, so a bunch gets the same bits from either.
@@ -524,9 +550,11 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
 // KM, how the kick reads P: kFullWindow (the LDS window holds the whole table:
 // no global gathers), kGatherFirst or kPrefetchFirst (a partial window, the
 // particles outside it gather from global memory; see the loop body).
+// CIC: the bunch's post-drift z are also deposited (cic_deposit) into the LDS
+// counters at byte cb.
 enum KickMode { kFullWindow = 0, kGatherFirst = 1, kPrefetchFirst = 2 };
-template <uint32_t FX, bool LH, int KM>
-__device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt1, int hb) {
+template <uint32_t FX, bool LH, int KM, bool CIC>
+__device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt1, int hb, int cb) {
     PassCtx c;
     load_ctx(B, c);
     const Flags<FX> fl{c.flags};
@@ -668,6 +696,10 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
                 bin_and_sum<LH>(c, ghist, z0, v);
                 bin_and_sum<LH>(c, ghist, z1, v);
             }
+            if (CIC) {
+                cic_deposit(c, cb, z0);
+                cic_deposit(c, cb, z1);
+            }
             v[2] += d0;
             v[2] += d1;
             // The prefetched pair is complete before this pair's stores are
@@ -696,6 +728,7 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             double d0;
             advance(c, fl, z0, e0, r0, d0);
             if (wake) bin_and_sum<LH>(c, ghist, z0, v);
+            if (CIC) cic_deposit(c, cb, z0);
             v[2] += d0;
             zp[i] = z0;
             ep[i] = e0;
@@ -715,6 +748,24 @@ __device__ inline void flush_hist(const BunchDev &B) {
         const uint32_t h = s_hist[b];
         if (h) __hip_atomic_fetch_add(g + b, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    if (B.flags & F_CIC) {   // and the deposit into this turn's counters
+        const int nb = B.nbins;
+        const unsigned long long *sH =
+            reinterpret_cast<const unsigned long long *>(pass_lds() + B.cic_off);
+        const uint32_t *sN = reinterpret_cast<const uint32_t *>(sH + 2 * nb);
+        g_u64 *gH = (g_u64 *)B.cic;
+        g_u32 *gN = (g_u32 *)(B.cic + 2 * nb);
+        for (int b = threadIdx.x; b <= nb; b += blockDim.x) {
+            const uint32_t cnt = sN[b];
+            if (!cnt) continue;
+            __hip_atomic_fetch_add(gN + b, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (b < nb) {
+                __hip_atomic_fetch_add(gH + b, sH[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(gH + nb + b, sH[nb + b], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
 }
 
 // Clears the LDS histogram of bunch B and loads its potential window.
@@ -723,6 +774,10 @@ __device__ inline void stage_bunch(const BunchDev &B) {
     const int hb = pass_hist_bytes(B);
     uint32_t *h = reinterpret_cast<uint32_t *>(pass_lds());
     for (int b = threadIdx.x; b < hb / 4; b += blockDim.x) h[b] = 0u;
+    if (B.flags & F_CIC) {
+        uint32_t *sC = reinterpret_cast<uint32_t *>(pass_lds() + B.cic_off);
+        for (int b = threadIdx.x; b < cic_bytes(B.nbins) / 4; b += blockDim.x) sC[b] = 0u;
+    }
     const TurnScalars &sc = *B.sc;
     if (sc.pending_kick && B.win > 0) {
         double *sP = reinterpret_cast<double *>(pass_lds() + hb);
@@ -763,16 +818,24 @@ __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bd
         // flags (only it has the registers), prefetch first in the generic one
         constexpr int kPartial = FX != 0 ? kGatherFirst : kPrefetchFirst;
         const bool full = B.win >= B.nbins;
+        const int cb = B.cic_off;
         if (kLdsOnly || hb) {
-            if (full)
-                turn_pass_run<FX, true, kFullWindow>(B, l0, l1, hb);
-            else
-                turn_pass_run<FX, true, kPartial>(B, l0, l1, hb);
+            if (B.flags & F_CIC) {   // the deposit needs the LDS histogram's room
+                if (full)
+                    turn_pass_run<FX, true, kFullWindow, true>(B, l0, l1, hb, cb);
+                else
+                    turn_pass_run<FX, true, kPartial, true>(B, l0, l1, hb, cb);
+            } else {
+                if (full)
+                    turn_pass_run<FX, true, kFullWindow, false>(B, l0, l1, hb, cb);
+                else
+                    turn_pass_run<FX, true, kPartial, false>(B, l0, l1, hb, cb);
+            }
         } else {
             if (full)
-                turn_pass_run<FX, false, kFullWindow>(B, l0, l1, hb);
+                turn_pass_run<FX, false, kFullWindow, false>(B, l0, l1, hb, cb);
             else
-                turn_pass_run<FX, false, kPrefetchFirst>(B, l0, l1, hb);
+                turn_pass_run<FX, false, kPrefetchFirst, false>(B, l0, l1, hb, cb);
         }
         tile = te;
     }
@@ -1518,7 +1581,7 @@ __device__ __forceinline__ void interp_sum_blocks(const BunchDev *__restrict__ b
     for (int tile = t0; tile < t1;) {
         const BunchDev &B = bds[tile_bunch[tile]];
         const int te = min(t1, B.tile_begin + B.ntiles);
-        if ((B.flags & F_WAKE) && (B.flags & F_E0))
+        if ((B.flags & F_WAKE) && (B.flags & F_E0) && !(B.flags & F_CIC))
             interp_sum_run<G>(B, tile - B.tile_begin, te - B.tile_begin);
         tile = te;
     }
@@ -1678,7 +1741,50 @@ __device__ inline void finish_turn(const BunchDev &B, double psum) {
     *B.sc = s;
 }
 
+// The sum over the bunch of the interpolated potential from the cloud-in-cell
+// deposit (F_CIC): sum_k (N_k - S_k) P[k] + S_k P[k+1] with
+// S_k = (hi_k 2^20 + lo_k) / 2^51, over the cells with particles (a cell
+// without particles adds nothing, even where P is not finite); NaN when some
+// particle's weight was not finite (as its interpolated value is).  Threads
+// [0, kRedThreads) take cells k = t + kRedThreads*j in ascending j and clear
+// them for the next turn; the waves' sums are added in wave order.  Valid in
+// thread 0; every thread of the workgroup calls it, and the sliced and the
+// whole-bunch finalize get the same bits from it.
+__device__ inline double cic_psum(const BunchDev &B, double *s_red) {
+    const int nb = B.nbins;
+    g_u64 *H = (g_u64 *)B.cic;
+    g_u32 *N = (g_u32 *)(B.cic + 2 * nb);
+    const g_f64 *P = (const g_f64 *)B.P;
+    if ((int)threadIdx.x < kRedThreads) {
+        double v = 0.0;
+        for (int k = (int)threadIdx.x; k < nb - 1; k += kRedThreads) {
+            const uint32_t cnt = N[k];
+            if (cnt) {
+                const double sk = ((double)H[k] * (double)(1 << kCicLoBits) + (double)H[nb + k]) *
+                                  (1.0 / kCicScale);
+                v += ((double)cnt - sk) * P[k] + sk * P[k + 1];
+                N[k] = 0u;
+                H[k] = 0ull;
+                H[nb + k] = 0ull;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kRedThreads / 64; ++w) t += s_red[w];
+        const uint32_t bad = N[nb];
+        N[nb] = 0u;
+        if (bad) t = __builtin_nan("");
+    }
+    return t;
+}
+
 // finish_turn after the sum of the interpolation sums, one workgroup per slice
+// (a cloud-in-cell bunch: its deposit's sum in slice 0, zero in the others)
 __global__ __launch_bounds__(kRedThreads) void k_finalize(const BunchDev *__restrict__ bds,
                                                           const int32_t *__restrict__ items) {
     __shared__ double s_red[16];
@@ -1686,19 +1792,33 @@ __global__ __launch_bounds__(kRedThreads) void k_finalize(const BunchDev *__rest
     const int r = items[2 * blockIdx.x + 1];
     const uint32_t fl = B.flags;
     double v[1] = {0.0};   // sum of the interpolated P (the sum of dE'-dE_init is in sc)
-    if ((fl & F_E0) && (fl & F_WAKE)) slice_sum<1>(B.ipart, B.ntiles * kTileWaves, r, v, s_red);
+    if ((fl & F_E0) && (fl & F_WAKE)) {
+        if (!(fl & F_CIC))
+            slice_sum<1>(B.ipart, B.ntiles * kTileWaves, r, v, s_red);
+        else if (r == 0)
+            v[0] = cic_psum(B, s_red);
+    }
     double t[1];
     if (!slices_done<1>(B, r, v, t) || threadIdx.x != 0) return;
     finish_turn(B, t[0]);
 }
 
-// k_finalize with one 1024-thread workgroup per bunch (whole_sum)
+// k_finalize with one 1024-thread workgroup per bunch (whole_sum; a
+// cloud-in-cell bunch's sum as slices_done adds slice 0's and the zeros)
 __global__ __launch_bounds__(1024) void k_finalize_whole(const BunchDev *__restrict__ bds,
                                                          const int32_t *__restrict__ list) {
+    __shared__ double s_red[16];
     const BunchDev &B = bds[list[blockIdx.x]];
     const uint32_t fl = B.flags;
     double t[1] = {0.0};
-    if ((fl & F_E0) && (fl & F_WAKE)) whole_sum<1>(B, B.ipart, t);
+    if ((fl & F_E0) && (fl & F_WAKE)) {
+        if (!(fl & F_CIC)) {
+            whole_sum<1>(B, B.ipart, t);
+        } else {
+            const double d = cic_psum(B, s_red);
+            t[0] = B.nred == 1 ? d : 0.0 + d;
+        }
+    }
     if (threadIdx.x == 0) finish_turn(B, t[0]);
 }
 

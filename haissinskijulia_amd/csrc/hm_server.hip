@@ -33,7 +33,7 @@ struct Job {
 size_t job_bytes(const hm_job &j) {
     const size_t n = (size_t)std::max<int64_t>(j.n, 1);
     const size_t nb = (size_t)std::max<int64_t>(j.nbins, 1);
-    return 16 * n + 96 * nb + (size_t)(1 << 16) + 2 * (size_t)kMaxFFT * 16;
+    return 16 * n + 104 * nb + (size_t)(1 << 16) + 2 * (size_t)kMaxFFT * 16;
 }
 
 }  // namespace

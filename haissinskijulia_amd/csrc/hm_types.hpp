@@ -41,7 +41,22 @@ enum BunchFlags : uint32_t {
     F_CHARGE = 64u,    // fixed wake current (charge extension / stage entry)
     F_GHIST = 128u,    // histogram straight to global memory (too many bins for LDS)
     F_LAG = 256u,      // wake table by lag behind the source: bin k feels bins k+j (causal)
+    F_CIC = 512u,      // the pass deposits the interpolation weights of the new z (cic)
 };
+
+// Cloud-in-cell deposit (F_CIC).  The E0 update needs sum_i interp(P_t, z_i)
+// over the post-drift z of turn t, with P_t known only after the histogram of
+// those z.  The interpolation is linear in P: with (k_i, w_i) the cell and
+// weight of z_i, the sum is sum_k (N_k - S_k) P_t[k] + S_k P_t[k+1], where
+// N_k counts the particles of cell k and S_k adds their weights.  The pass
+// counts N_k and adds W_i = round(w_i 2^51) into two 64-bit counters per cell
+// (W >> 20 and the low 20 bits): integer sums, so they do not depend on the
+// order of the additions, and 2^-52 of a weight is as fine as the weight's
+// own FP64 rounding.  Layout, in LDS and in global memory: hi[nbins],
+// lo[nbins] (64-bit), then n[nbins + 1] (32-bit; n[nbins] counts particles
+// whose weight is not finite).
+constexpr double kCicScale = 2251799813685248.0;   // 2^51
+constexpr int kCicLoBits = 20;
 
 // the all-physics-on configuration the turn pass is specialised for, and the
 // flags that select the pass's code path
@@ -105,16 +120,28 @@ struct BunchDev {
     double *part, *ipart, *moments, *out;
     double *red;          // [nred][4]
     uint32_t *cnt;
-    int32_t nred, pad_red;
+    int32_t nred;
+    int32_t cic_off;      // F_CIC: byte offset of the pass's LDS deposit
     TurnScalars *sc;
     double *mon;          // device [n_turns][4] turn monitor or null
+    unsigned long long *cic;   // F_CIC: this turn's deposit (hi, lo, n; zero between turns)
 };
 
 // LDS of a turn-pass workgroup for one bunch: its histogram (nbins counts,
 // unless F_GHIST), then a window of `win` entries of the previous turn's
-// potential, P[lo .. lo+win), for the wake-kick gathers.
+// potential, P[lo .. lo+win), for the wake-kick gathers, then with F_CIC the
+// deposit's counters at cic_off.
 __host__ __device__ inline int pass_hist_bytes(const BunchDev &B) {
     return ((B.flags & F_WAKE) && !(B.flags & F_GHIST)) ? ((B.nbins * 4 + 15) & ~15) : 0;
+}
+
+__host__ __device__ inline int cic_bytes(int nbins) { return (16 * nbins + 4 * (nbins + 1) + 15) & ~15; }
+__host__ __device__ inline int pass_cic_bytes(const BunchDev &B) {
+    return (B.flags & F_CIC) ? cic_bytes(B.nbins) : 0;
+}
+// the pass's dynamic LDS for one bunch
+__host__ __device__ inline int pass_lds_bytes(const BunchDev &B) {
+    return (B.flags & F_CIC) ? B.cic_off + pass_cic_bytes(B) : pass_hist_bytes(B) + 8 * B.win;
 }
 
 // first entry of a window of w entries of P centred on `centre`

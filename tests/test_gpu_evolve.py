@@ -143,7 +143,9 @@ def test_pass_variants_agree_bitwise(dev, nbins):
     # the device's own normal stream (draw_normals, injected): the same
     # arithmetic, so the same bits.  Bins over +-2 sigma_z0 leave one particle
     # in seven outside the LDS window (+-1.5 sigma_z0) at 16384 bins: gather
-    # first against prefetch first.  At 8192 bins the window holds the table.
+    # first against prefetch first.  At 8192 bins the pass also deposits the
+    # interpolation weights (cloud in cell), whose counters leave room for a
+    # window of 8062 of the 8192 entries: the same two orders of loads.
     n, turns, seed = 20_000, 30, 101
     p = to_sim_params(W.ring_3gev_lattice(n_turns=turns))
     z0, e0 = O.generate_gaussian(1, 0.0, 4e-3, 0.0, 3e6, n)

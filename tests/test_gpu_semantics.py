@@ -144,3 +144,27 @@ def test_lag_wake_potential_is_causal_on_the_gpu():
     print(f"max |kick| ahead {np.abs(kick[ahead]).max():.3e}, "
           f"min |kick| behind {np.abs(kick[behind]).min():.3e}", flush=True)
     assert np.abs(kick[ahead]).max() <= 0.02 * np.abs(kick[behind]).min()
+
+
+@pytest.mark.parametrize("nbins", [2, 1024, 8192])
+def test_cloud_in_cell_sum_with_particles_off_the_grid(nbins):
+    """With update_E0, a bunch of at most 8192 bins takes the sum of the
+    interpolated potential (evolution.jl:419-429) from the pass's cloud-in-cell
+    deposit instead of a second read of z.  Particles beyond the end centres
+    clamp to the end cells as the kick's interpolation does
+    (wakefield.jl:64-78): bins over +-1.6 sigma_z0 leave a tenth of the
+    bunch outside the grid.  Two bins are the smallest grid with a cell."""
+    p = W.ring_3gev_lattice(30)
+    n = 20_001
+    z0, e0 = O.generate_gaussian(12, 0.0, 4e-3, 0.0, 3e6, n)
+    assert np.mean(np.abs(z0) > 1.6 * 4e-3) > 0.09
+    kw = dict(resonators=[W.RESONATOR_BB], charge=5e-9, bin_half_width_sigmas=1.6)
+    nrm = np.vstack([O.normals(13, t, 0, n) for t in range(30)])
+    zr, er, rr = O.evolve(z0, e0, nbins, p, normals=nrm, **kw)
+    assert rr.status == 0
+    z, e, (sE, sz, E0) = _evolve(z0, e0, nbins, p, normals=nrm, **kw)
+    print(f"nbins {nbins}: normwise z={normwise(z, zr):.2e} dE={normwise(e, er):.2e} "
+          f"E0 {E0!r} oracle {rr.E0!r}", flush=True)
+    assert normwise(z, zr) <= COORD_RTOL and normwise(e, er) <= COORD_RTOL
+    assert E0 == pytest.approx(rr.E0, rel=1e-14)
+    assert sE == pytest.approx(rr.sigma_E, rel=1e-10) and sz == pytest.approx(rr.sigma_z, rel=1e-10)
