@@ -669,6 +669,7 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         d.e_last = zw;
         d.inv_step = 1.0 / step;                                              // utils.jl:378
         d.inv_s = nb > 1 ? 1.0 / ((d.c_last - d.c_first) / (double)(nb - 1)) : 0.0;  // wakefield.jl:39-42
+        d.u_last = (d.c_last - d.c_first) * d.inv_s;
         std::vector<double> W, lam;
         const double wf = ((120.0 * kPi) * kSpeedLight) / (kPi * q.p.pipe_radius);   // evolution.jl:279
         const double ws = std::sqrt(jl_max_h(0.0, (2.0 * 30.0) / q.p.pipe_radius));  // evolution.jl:280

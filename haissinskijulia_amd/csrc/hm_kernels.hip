@@ -170,19 +170,28 @@ __device__ __forceinline__ void qe_pair_normals(uint64_t seed, uint64_t p, uint3
 }
 
 // zero_alloc_interpolation! index and weight (wakefield.jl:38-42,64-78) for
-// nb >= 2.  The clamp of z to [c0, c1] is one v_max_f64 and one v_min_f64,
-// which map a NaN z to c0; the reference's clamp keeps the NaN, its weight
-// u - floor(u) is NaN and so is the interpolated value (cell 0 for the index,
-// as the clamp of a converted NaN gives).  interp_index_pass leaves that
-// select out: in the turn pass a NaN-z particle's dE turns NaN through the RF
-// term of the same pass whatever its kick, so no result depends on it.
-__device__ __forceinline__ void interp_index_pass(double z, double c0, double c1, double inv,
+// nb >= 2.  interp_index clamps z to [c0, c1] by one v_max_f64 and one
+// v_min_f64, which map a NaN z to c0; the reference's clamp keeps the NaN, its
+// weight u - floor(u) is NaN and so is the interpolated value (cell 0 for the
+// index, as the clamp of a converted NaN gives), which interp_index selects
+// back.
+//
+// interp_index_pass, the turn pass's form, clamps the coordinate
+// u = (z - c0) * inv to [0, u1], u1 = (c1 - c0) * inv, instead of z to
+// [c0, c1]: the same u for every z that is not NaN, since rounding is
+// monotonic (z > c1 gives u >= u1, z < c0 gives u <= 0, and the clamped
+// z = c0 gives +0).  u is an arithmetic result, so the clamp needs no
+// canonicalising of z and the ends, and it is never NaN (max with 0 drops a
+// NaN): a NaN z lands on cell 0 with weight 0, and k = min(int(floor(u)),
+// nb - 2) lies in [0, nb - 2] without a NaN select or a lower bound.  In the
+// turn pass a NaN-z particle's dE turns NaN through the RF term of the same
+// pass whatever its kick, so no result depends on that weight (the
+// cloud-in-cell deposit counts a NaN z apart).
+__device__ __forceinline__ void interp_index_pass(double z, double c0, double u1, double inv,
                                                   int nb, int &k, double &w) {
-    const double cl = __builtin_fmin(__builtin_fmax(z, c0), c1);
-    const double u = (cl - c0) * inv;
+    const double u = __builtin_fmin(__builtin_fmax((z - c0) * inv, 0.0), u1);
     const double fu = floor(u);
-    const int kk = (fu == fu) ? (int)fu : 0;
-    k = min(max(kk, 0), nb - 2);
+    k = min((int)fu, nb - 2);
     w = u - fu;
 }
 __device__ __forceinline__ void interp_index(double z, double c0, double c1, double inv, int nb,
@@ -364,7 +373,7 @@ struct PassCtx {
     int pending_kick, pending_m, drift_skip;
     uint32_t flags;
     const double *P;
-    double c0, c1, inv_s, e0, e1, inv_step;
+    double c0, c1, u1, inv_s, e0, e1, inv_step;
     int nb;
 };
 
@@ -390,6 +399,7 @@ __device__ inline void load_ctx(const BunchDev &B, PassCtx &c) {
     c.P = B.P;
     c.c0 = B.c_first;
     c.c1 = B.c_last;
+    c.u1 = B.u_last;
     c.inv_s = B.inv_s;
     c.e0 = B.e_first;
     c.e1 = B.e_last;
@@ -521,7 +531,7 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
 __device__ __forceinline__ void cic_deposit(const PassCtx &c, int cb, double z) {
     int k;
     double w;
-    interp_index_pass(z, c.c0, c.c1, c.inv_s, c.nb, k, w);
+    interp_index_pass(z, c.c0, c.u1, c.inv_s, c.nb, k, w);
     unsigned long long *sH = reinterpret_cast<unsigned long long *>(pass_lds() + cb);
     unsigned long long *sL = sH + c.nb;
     uint32_t *sN = reinterpret_cast<uint32_t *>(sL + c.nb);
@@ -639,8 +649,8 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             double2 q0, q1;
             if (kick2) {
                 int k0, k1;
-                interp_index_pass(zz.x, c.c0, c.c1, c.inv_s, c.nb, k0, w0);
-                interp_index_pass(zz.y, c.c0, c.c1, c.inv_s, c.nb, k1, w1);
+                interp_index_pass(zz.x, c.c0, c.u1, c.inv_s, c.nb, k0, w0);
+                interp_index_pass(zz.y, c.c0, c.u1, c.inv_s, c.nb, k1, w1);
                 const unsigned j0 = (unsigned)(k0 - wlo), j1 = (unsigned)(k1 - wlo);
                 // a full window, and any window of the all-flags pass (its
                 // histogram is in LDS, so its window has at least two entries),
@@ -1756,16 +1766,40 @@ __device__ inline double cic_psum(const BunchDev &B, double *s_red) {
     g_u32 *N = (g_u32 *)(B.cic + 2 * nb);
     const g_f64 *P = (const g_f64 *)B.P;
     if ((int)threadIdx.x < kRedThreads) {
+        // U cells per thread at a time, every load of the group issued before
+        // its arithmetic (the loads of empty cells read zeros and are not used)
+        constexpr int U = 8;
         double v = 0.0;
-        for (int k = (int)threadIdx.x; k < nb - 1; k += kRedThreads) {
-            const uint32_t cnt = N[k];
-            if (cnt) {
-                const double sk = ((double)H[k] * (double)(1 << kCicLoBits) + (double)H[nb + k]) *
-                                  (1.0 / kCicScale);
-                v += ((double)cnt - sk) * P[k] + sk * P[k + 1];
-                N[k] = 0u;
-                H[k] = 0ull;
-                H[nb + k] = 0ull;
+        for (int k0 = (int)threadIdx.x; k0 < nb - 1; k0 += U * kRedThreads) {
+            uint32_t cnt[U];
+            unsigned long long hi[U], lo[U];
+            double p0[U], p1[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * kRedThreads;
+                const bool in = k < nb - 1;
+                cnt[u] = in ? N[k] : 0u;
+                hi[u] = in ? H[k] : 0ull;
+                lo[u] = in ? H[nb + k] : 0ull;
+                p0[u] = in ? P[k] : 0.0;
+                p1[u] = in ? P[k + 1] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (cnt[u]) {
+                    const double sk = ((double)hi[u] * (double)(1 << kCicLoBits) + (double)lo[u]) *
+                                      (1.0 / kCicScale);
+                    v += ((double)cnt[u] - sk) * p0[u] + sk * p1[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u * kRedThreads;
+                if (cnt[u]) {
+                    N[k] = 0u;
+                    H[k] = 0ull;
+                    H[nb + k] = 0ull;
+                }
             }
         }
 #pragma unroll

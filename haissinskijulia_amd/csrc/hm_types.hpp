@@ -106,6 +106,7 @@ struct BunchDev {
     // histogram / interpolation grids (evolution.jl:270-274, wakefield.jl:38-42)
     double e_first, e_last, inv_step;
     double c_first, c_last, inv_s;
+    double u_last;        // (c_last - c_first) * inv_s: the interpolation coordinate of the last centre
     uint32_t *hist;       // [2][nbins]: half (turn & 1) is this turn's
     double *P;
     double *P2;           // interleaved pairs (P[k], P[k+1]), k < nbins (P[nbins] := 0)
