@@ -391,7 +391,9 @@ __device__ inline void load_ctx(const BunchDev &B, PassCtx &c) {
     c.alpha_c = B.alpha_c;
     c.kcoef = s.kcoef;
     c.coeff_c = s.coeff_c;
-    c.m_prev = s.m_prev;
+    // 0 when no re-centre is pending: subtracting it then changes nothing
+    // (x - 0 = x for every x, -0 and NaN included)
+    c.m_prev = s.pending_m ? s.m_prev : 0.0;
     c.pending_kick = s.pending_kick;
     c.pending_m = s.pending_m;
     c.drift_skip = s.drift_skip;
@@ -512,8 +514,10 @@ __device__ __forceinline__ void bin_and_sum(const PassCtx &c, g_u32 *ghist, doub
                                             double (&v)[4]) {
     v[0] += z;
     v[1] += z * z;
-    const int b = hist_bin(z, c.e0, c.e1, c.inv_step, c.nb);
-    if (b >= 0) {
+    // hist_bin with its range test as the branch: in range, u = (z - e0) * inv
+    // is >= 0, and its conversion to int (truncation) is floor(u)
+    if ((z >= c.e0) & (z <= c.e1)) {
+        const int b = min((int)((z - c.e0) * c.inv_step), c.nb - 1);
         if (LH)
             atomicAdd(reinterpret_cast<uint32_t *>(pass_lds()) + b, 1u);
         else
@@ -690,7 +694,10 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
                     e1 = e1 - c.P[0];
                 }
             }
-            if (c.pending_m) {
+            // with update_E0 every turn but the first has a re-centre pending,
+            // and m_prev is 0 when none is: the all-flags pass subtracts it
+            // without a select
+            if (fl.has(F_E0) || c.pending_m) {
                 e0 = e0 - c.m_prev;
                 e1 = e1 - c.m_prev;
             }
