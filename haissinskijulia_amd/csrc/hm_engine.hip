@@ -410,7 +410,11 @@ hm_status Batch::build_tables() {
         const BunchDev &d = rb_[(size_t)tiles[(size_t)t]].d;
         const int64_t lt = t - d.tile_begin;
         const int64_t pairs = std::min(d.pairs_per_tile, d.npairs - lt * d.pairs_per_tile);
-        wpass[(size_t)t] = pairs;
+        // a run of a bunch's tiles first stages the bunch's LDS (histogram,
+        // deposit, window of P) and ends by flushing it: about as long as
+        // streaming 256 + nbins/4 pairs, charged to the bunch's first tile
+        const int64_t run = (lt == 0 && (d.flags & F_WAKE)) ? 256 + (int64_t)d.nbins / 4 : 0;
+        wpass[(size_t)t] = pairs + run;
         winterp[(size_t)t] = ((d.flags & F_WAKE) && (d.flags & F_E0) && !(d.flags & F_CIC)) ? pairs : 0;
     }
     const int per_cu = interp_lds_ + 1024 <= (size_t)ctx_->lds_max / 2 ? 2 : 1;
