@@ -161,6 +161,8 @@ class TrackingOptions:
     monitor: bool = False                # record per-turn centroid, sigma_z, E0, binned count
     serial: bool = False                 # the use_mpi = false branch (longitudinal_evolve sets it)
     fom_stats: bool = False              # final two-pass means and cor(z, ΔE) (figures of merit)
+    tile_pairs: int = 0                  # pairs per thread per tile of the turn pass (0: tuned/rule)
+    run_pairs: int = 0                   # cost of a run of tiles when balancing the pass (0: tuned/rule)
 
     def _c(self, keep: list, n: int, n_turns: int, nbins: int) -> HmOptions:
         o = HmOptions()
@@ -197,6 +199,10 @@ class TrackingOptions:
             o._monitor = mon
         o.serial = int(bool(self.serial))
         o.fom_stats = int(bool(self.fom_stats))
+        # 0: the tuned values for this bunch size and bin count (tuning.json),
+        # else the library's rules
+        o.tile_pairs = int(self.tile_pairs) or _tune.tile_pairs_for(n)
+        o.run_pairs = int(self.run_pairs) or _tune.run_pairs_for(nbins)
         return o
 
 

@@ -264,6 +264,8 @@ hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t n
     if (o && o->n_resonators > 0 && !o->resonators) return fail(HM_ERR_ARG, "resonators is NULL");
     if (o && o->block_size && o->block_size != kPassThreads)
         return fail(HM_ERR_ARG, "block_size must be 0 or %d", kPassThreads);
+    if (o && (o->tile_pairs < 0 || o->tile_pairs > (1 << 20) || o->run_pairs < 0))
+        return fail(HM_ERR_ARG, "tile_pairs must be in [0, 2^20] and run_pairs >= 0");
     return HM_OK;
 }
 
@@ -271,12 +273,13 @@ hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t n
 // Batch
 // ---------------------------------------------------------------------------
 
-static int64_t pairs_per_tile_for(int64_t npairs, int threads) {
-    // tiling depends on the bunch size alone, so a bunch is reduced in the
-    // same order alone and in any batch: about 1024 tiles of a large bunch
-    // (four per CU), at least one pair per thread.  A workgroup streams its
-    // run of consecutive tiles of a bunch as one loop, so small tiles cost
-    // no extra pipeline start.
+static int64_t pairs_per_tile_for(int64_t npairs, int threads, int32_t tile_pairs) {
+    // tiling depends on the bunch size (and the option) alone, so a bunch is
+    // reduced in the same order alone and in any batch: by default about 1024
+    // tiles of a large bunch (four per CU), at least one pair per thread.  A
+    // workgroup streams its run of consecutive tiles of a bunch as one loop,
+    // so small tiles cost no extra pipeline start.
+    if (tile_pairs > 0) return (int64_t)tile_pairs * threads;
     const int64_t target_tiles = 1024;
     int64_t ppt = (npairs + target_tiles - 1) / target_tiles;
     ppt = ((ppt + threads - 1) / threads) * threads;
@@ -413,7 +416,9 @@ hm_status Batch::build_tables() {
         // a run of a bunch's tiles first stages the bunch's LDS (histogram,
         // deposit, window of P) and ends by flushing it: about as long as
         // streaming 256 + nbins/4 pairs, charged to the bunch's first tile
-        const int64_t run = (lt == 0 && (d.flags & F_WAKE)) ? 256 + (int64_t)d.nbins / 4 : 0;
+        const RunBunch &q = rb_[(size_t)tiles[(size_t)t]];
+        const int64_t cost = q.o.run_pairs > 0 ? q.o.run_pairs : 256 + (int64_t)d.nbins / 4;
+        const int64_t run = (lt == 0 && (d.flags & F_WAKE)) ? cost : 0;
         wpass[(size_t)t] = pairs + run;
         winterp[(size_t)t] = ((d.flags & F_WAKE) && (d.flags & F_E0) && !(d.flags & F_CIC)) ? pairs : 0;
     }
@@ -549,7 +554,7 @@ hm_status Batch::add_slots(const std::vector<RunBunch> &specs, std::vector<int32
         d.dE = b->dE;
         d.n = b->n;
         d.npairs = (b->n + 1) / 2;
-        d.pairs_per_tile = pairs_per_tile_for(d.npairs, kPassThreads);
+        d.pairs_per_tile = pairs_per_tile_for(d.npairs, kPassThreads, q.o.tile_pairs);
         d.ntiles = (int32_t)((d.npairs + d.pairs_per_tile - 1) / d.pairs_per_tile);
         uint32_t fl = 0;
         if (p->use_wakefield) fl |= F_WAKE;

@@ -1,18 +1,34 @@
-"""Offline tuner of the convolution block size (bins per FFT block).
+"""Offline tuner of the library's launch knobs, written to ``tuning.json``.
 
-``python -m haissinskijulia_amd.tune`` times the wake convolution of one
-turn (``TrackingOptions(timing=True)``, HIP events around the convolution
-launches) for every admissible block size at each bin count and writes the
-fastest into ``tuning.json`` beside this file.  ``TrackingOptions`` with
-``fft_block=0`` then takes the block from that table; bin counts it does not
-list keep the library's own rule.  The table is read once and depends only on
-the bin count, so a bunch gets the same block, and the same bits, alone, in a
-batch and in a served mix.
+Three tables, each timed on the device with HIP events:
+
+- ``fft_block``: bins per FFT block of the wake convolution, by bin count
+  (the convolution's own time per turn, ``TrackingOptions(timing=True)``);
+- ``tile_pairs``: particle pairs per thread in each tile of the turn pass,
+  by bunch size class (round(log2 of the pair count)), from the whole
+  evolve call's device time (the tiling sets the pass's work per
+  workgroup, the number of per-wave partial sums and the reductions over
+  them);
+- ``run_pairs``: the cost, in pairs, that the pass's workgroup balancing
+  charges to a run of one bunch's tiles (staging and flushing its LDS), by
+  bin count, from the batched 256-bunch scan's device time.
+
+The threads per block of the turn pass are not a knob: its workgroup holds
+the LDS histogram and the window of the potential and takes the whole LDS
+of a CU, so it is one 1024-thread workgroup per CU.
+
+``python -m haissinskijulia_amd.tune --write`` runs all three and writes
+the table beside this file; ``TrackingOptions`` fields left at 0 take their
+values from it, and keys it does not list keep the library's own rules.
+Each value depends only on the bin count or the bunch size, so a bunch gets
+the same tiling, and the same bits, alone, in a batch and in a served mix
+(the run cost and the FFT block change no result at all).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 
@@ -22,19 +38,53 @@ TABLE = os.path.join(HERE, "tuning.json")
 _table = None
 
 
-def fft_block_for(nbins: int) -> int:
-    """The tuned convolution block for `nbins`, 0 if the table has none."""
+def _tables() -> dict:
     global _table
     if _table is None:
         try:
             with open(TABLE) as f:
-                _table = {int(k): int(v) for k, v in json.load(f).get("fft_block", {}).items()}
-        except (OSError, ValueError):
-            _table = {}
-    return _table.get(int(nbins), 0)
+                raw = json.load(f)
+            _table = {name: {int(k): int(v) for k, v in raw.get(name, {}).items()}
+                      for name in ("fft_block", "tile_pairs", "run_pairs")}
+        except (OSError, ValueError, AttributeError):
+            _table = {"fft_block": {}, "tile_pairs": {}, "run_pairs": {}}
+    return _table
 
 
-def tune(nbins_list, n: int = 200_000, turns: int = 60, reps: int = 3) -> dict:
+def fft_block_for(nbins: int) -> int:
+    """The tuned convolution block for `nbins`, 0 if the table has none."""
+    return _tables()["fft_block"].get(int(nbins), 0)
+
+
+def size_class(n: int) -> int:
+    """The bunch-size class of the tile table: round(log2(pairs))."""
+    pairs = (int(n) + 1) // 2
+    return int(round(math.log2(pairs))) if pairs > 0 else 0
+
+
+def tile_pairs_for(n: int) -> int:
+    """The tuned pairs per thread per tile for a bunch of n particles: the
+    entry of the nearest tuned size class (at most two classes away), 0 if
+    there is none."""
+    tab = _tables()["tile_pairs"]
+    if not tab or n <= 0:
+        return 0
+    c = size_class(n)
+    best = min(tab, key=lambda k: (abs(k - c), k))
+    return tab[best] if abs(best - c) <= 2 else 0
+
+
+def run_pairs_for(nbins: int) -> int:
+    """The tuned run cost for bunches of `nbins` bins, 0 if the table has none."""
+    return _tables()["run_pairs"].get(int(nbins), 0)
+
+
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2]
+
+
+def tune_fft(nbins_list, n: int = 200_000, turns: int = 60, reps: int = 3) -> dict:
     import haissinskijulia_amd as H
     from haissinskijulia_amd import workloads as W
     dev = H.MI355X.default()
@@ -56,7 +106,7 @@ def tune(nbins_list, n: int = 200_000, turns: int = 60, reps: int = 3) -> dict:
                                                  timing=True, fft_block=B))
                 b.close()
                 ms.append(res.ms_conv_avg)
-            t = sorted(ms)[len(ms) // 2]
+            t = _median(ms)
             print(f"nbins {nb:6d} block {B:5d}: convolution {t * 1e3:8.2f} us", flush=True)
             if best is None or t < best[1]:
                 best = (B, t)
@@ -64,20 +114,95 @@ def tune(nbins_list, n: int = 200_000, turns: int = 60, reps: int = 3) -> dict:
     return out
 
 
+def tune_tiles(sizes, nbins: int = 4096, turns: int = 40, reps: int = 3) -> dict:
+    """Pairs per thread per tile by bunch size class: the whole evolve call's
+    device time (set-up, turns and epilogue) for each candidate, the library's
+    rule (0) among them; the rule is kept unless a candidate beats it by more
+    than 1%."""
+    import haissinskijulia_amd as H
+    from haissinskijulia_amd import workloads as W
+    dev = H.MI355X.default()
+    out = {}
+    for n in sizes:
+        n = int(n)
+        pairs = (n + 1) // 2
+        cands = [0] + [p for p in (1, 2, 4, 8, 16, 32, 64, 128) if p * 1024 <= max(pairs, 1024)]
+        b = H.DeviceBunch(n, dev)
+        times = {}
+        for tp in cands:
+            ms = []
+            for r in range(reps):
+                b.generate(1 + r, W.SIGMA_Z, W.SIGMA_E)
+                res = b.evolve(H.SimulationParameters.from_dict(W.ring_3gev_lattice(turns)), nbins,
+                               H.TrackingOptions(seed=2, resonators=[W.RESONATOR_BB], charge=5e-9,
+                                                 bin_half_width_sigmas=8.0, tile_pairs=tp or 0,
+                                                 fft_block=fft_block_for(nbins)))
+                ms.append(res.ms_total)
+            times[tp] = _median(ms)
+            print(f"n {n:10d} tile_pairs {tp or 'rule':>4}: {times[tp] / turns * 1e3:9.2f} us/turn",
+                  flush=True)
+        b.close()
+        best = min(times, key=times.get)
+        if best != 0 and times[best] < 0.99 * times[0]:
+            out[size_class(n)] = best
+    return out
+
+
+def tune_runs(candidates=(64, 256, 512, 1024, 1280, 2048, 4096, 8192), turns: int = 40,
+              reps: int = 3) -> dict:
+    """Run cost for the scan's 4096-bin bunches: the batched 256-bunch scan's
+    device time per candidate (the library's rule, 256 + nbins/4 = 1280, among
+    them)."""
+    import haissinskijulia_amd as H
+    from haissinskijulia_amd import workloads as W
+    dev = H.MI355X.default()
+    specs = W.scan_bunches()
+    nbins = specs[0]["nbins"]
+    bunches = [H.DeviceBunch(s["n"], dev) for s in specs]
+    params = [H.SimulationParameters.from_dict(W.ring_3gev_lattice(turns, voltage=s["V"]))
+              for s in specs]
+    times = {}
+    for c in candidates:
+        ms = []
+        for r in range(reps):
+            for s, b in zip(specs, bunches):
+                b.generate(s["seed"], W.SIGMA_Z, W.SIGMA_E)
+            opts = [H.TrackingOptions(seed=s["seed"] + 1, resonators=s["resonators"],
+                                      charge=s["charge"], bin_half_width_sigmas=s["half_width"],
+                                      run_pairs=c) for s in specs]
+            rs = H.batch_evolve(bunches, [s["nbins"] for s in specs], params, opts)
+            ms.append(rs[0].ms_total)
+        times[c] = _median(ms)
+        print(f"scan, {nbins} bins, run_pairs {c:5d}: {times[c] / turns * 1e3:9.2f} us/turn",
+              flush=True)
+    for b in bunches:
+        b.close()
+    return {nbins: min(times, key=times.get)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--what", nargs="*", default=["fft", "tiles", "runs"],
+                    choices=["fft", "tiles", "runs"])
     ap.add_argument("--nbins", type=int, nargs="*",
                     default=[256, 512, 1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072])
-    ap.add_argument("--write", action="store_true", help="write tuning.json")
+    ap.add_argument("--sizes", type=float, nargs="*", default=[1e4, 1e5, 1e6, 1e7, 1e8],
+                    help="bunch sizes of the tile sweep")
+    ap.add_argument("--write", action="store_true", help="write the table")
     ap.add_argument("--out", default=TABLE, help="where --write puts the table")
     args = ap.parse_args()
-    table = tune(args.nbins)
-    print(json.dumps({"fft_block": table}))
+    table = {name: {str(k): v for k, v in sorted(t.items())} for name, t in _tables().items()}
+    if "fft" in args.what:
+        table["fft_block"] = {str(k): v for k, v in sorted(tune_fft(args.nbins).items())}
+    if "tiles" in args.what:
+        table["tile_pairs"] = {str(k): v for k, v in sorted(tune_tiles(args.sizes).items())}
+    if "runs" in args.what:
+        table["run_pairs"] = {str(k): v for k, v in sorted(tune_runs().items())}
+    print(json.dumps(table))
     if args.write:
         import haissinskijulia_amd as H
         with open(args.out, "w") as f:
-            json.dump({"device": H.MI355X.default().info()["arch"],
-                       "fft_block": {str(k): v for k, v in sorted(table.items())}}, f, indent=1)
+            json.dump({"device": H.MI355X.default().info()["arch"], **table}, f, indent=1)
     return 0
 
 

@@ -85,6 +85,15 @@ typedef struct {
     int32_t fom_stats;          /* 1: the final two-pass means and Statistics.cor(z, dE) in
                                    hm_result (mean_z, mean_dE, corr_z_dE: two more reads of the
                                    bunch); 0: corr_z_dE is NaN and the means are one-pass */
+    int32_t tile_pairs;         /* particle pairs per thread in each tile of the turn pass (a
+                                   tile is tile_pairs x block size pairs); 0 = the library's
+                                   rule (about 1024 tiles per bunch, at least one pair per
+                                   thread).  The per-tile sums follow the tiling, so runs of a
+                                   bunch agree bit for bit under the same value */
+    int32_t run_pairs;          /* the cost, in pairs, charged to a run of this bunch's tiles
+                                   (staging and flushing its LDS) when the turn pass's
+                                   workgroup ranges are balanced; 0 = 256 + nbins/4.  Changes
+                                   no result */
 } hm_options;
 
 /* Return of longitudinal_evolve! (evolution.jl:470) plus diagnostics. */

@@ -283,3 +283,45 @@ def test_failed_admission_releases_its_slots():
         b.close()
         assert (res[jid].result.sigma_E, res[jid].result.sigma_z, res[jid].result.E0) == \
                (ra.sigma_E, ra.sigma_z, ra.E0)
+
+
+def test_tile_knobs_keep_bits_alone_and_batched():
+    """tile_pairs sets a bunch's tiling, hence the order of its per-tile sums:
+    under the same value a bunch gives the same bits alone and batched, and
+    every value stays within the parity bar of the rule's result (the oracle's
+    distance to either is rounding).  run_pairs only moves the pass's
+    workgroup boundaries: the same bits whatever its value."""
+    specs = [dict(n=300_001, nbins=1024, V=3.2e6, charge=3e-9, seed=71, flags=(True,) * 5),
+             dict(n=40_000, nbins=4096, V=3.6e6, charge=6e-9, seed=72, flags=(True,) * 5)]
+    turns = 20
+    runs = {}
+    for tp, rp in ((0, 0), (3, 0), (3, 1), (3, 1 << 20), (16, 0)):
+        bunches = [_bunch(s) for s in specs]
+        opts = []
+        for s in specs:
+            o = _opts(s)
+            o.tile_pairs, o.run_pairs = tp, rp
+            opts.append(o)
+        rs = H.batch_evolve(bunches, [s["nbins"] for s in specs],
+                            [_params(turns, s["V"], s["flags"]) for s in specs], opts)
+        runs[(tp, rp)] = [(r.E0, *b.download()) for r, b in zip(rs, bunches)]
+        for b in bunches:
+            b.close()
+        print(f"tile_pairs {tp} run_pairs {rp}: E0 {[r.E0 for r in rs]}", flush=True)
+    # alone under tile_pairs = 3
+    s = specs[0]
+    b = _bunch(s)
+    o = _opts(s)
+    o.tile_pairs = 3
+    r = b.evolve(_params(turns, s["V"], s["flags"]), s["nbins"], o)
+    za, ea = b.download()
+    b.close()
+    E0b, zb, eb = runs[(3, 0)][0]
+    assert r.E0 == E0b and np.array_equal(za, zb) and np.array_equal(ea, eb)
+    for rp in (1, 1 << 20):   # the run cost changes nothing
+        for (E0x, zx, ex), (E0y, zy, ey) in zip(runs[(3, 0)], runs[(3, rp)]):
+            assert E0x == E0y and np.array_equal(zx, zy) and np.array_equal(ex, ey)
+    for tp in (3, 16):        # another tiling: another order of the sums, the same physics
+        for (E0x, zx, ex), (E0y, zy, ey) in zip(runs[(0, 0)], runs[(tp, 0)]):
+            assert E0y == pytest.approx(E0x, rel=1e-14)
+            assert normwise(zy, zx) <= COORD_RTOL and normwise(ey, ex) <= COORD_RTOL

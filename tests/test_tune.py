@@ -38,3 +38,34 @@ def test_options_take_the_tuned_block(tmp_path, monkeypatch):
     assert o.fft_block == 256
     assert H.TrackingOptions(fft_block=1024)._c(keep, 1000, 10, 2048).fft_block == 1024
     assert H.TrackingOptions()._c(keep, 1000, 10, 512).fft_block == 0
+
+
+def test_tile_and_run_tables(tmp_path, monkeypatch):
+    path = tmp_path / "tuning.json"
+    path.write_text(json.dumps({"fft_block": {}, "tile_pairs": {"16": 2, "26": 4},
+                                "run_pairs": {"4096": 2048}}))
+    monkeypatch.setattr(tune, "TABLE", str(path))
+    monkeypatch.setattr(tune, "_table", None)
+    assert tune.size_class(2 * 65536) == 16 and tune.size_class(1) == 0
+    assert tune.tile_pairs_for(2 * 65536) == 2          # class 16
+    assert tune.tile_pairs_for(2 * 2 ** 18) == 2         # class 18: 16 is two classes away
+    assert tune.tile_pairs_for(2 * 2 ** 25) == 4         # class 25: 26 is the nearest
+    assert tune.tile_pairs_for(2 * 2 ** 21) == 0         # class 21: both five classes away
+    assert tune.tile_pairs_for(2 * 2 ** 20) == 0         # class 20: nothing within two classes
+    assert tune.tile_pairs_for(0) == 0
+    assert tune.run_pairs_for(4096) == 2048 and tune.run_pairs_for(1024) == 0
+    keep = []
+    o = H.TrackingOptions()._c(keep, 2 * 65536, 10, 4096)
+    assert (o.tile_pairs, o.run_pairs) == (2, 2048)
+    o = H.TrackingOptions(tile_pairs=8, run_pairs=100)._c(keep, 2 * 65536, 10, 4096)
+    assert (o.tile_pairs, o.run_pairs) == (8, 100)
+
+
+def test_committed_table_is_well_formed():
+    with open(tune.TABLE) as f:
+        table = json.load(f)
+    for name in ("fft_block", "tile_pairs", "run_pairs"):
+        for k, v in table.get(name, {}).items():
+            assert int(k) >= 0 and int(v) > 0
+    for k, v in table.get("fft_block", {}).items():
+        assert int(v) & (int(v) - 1) == 0                # powers of two
