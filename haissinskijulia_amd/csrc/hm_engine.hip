@@ -474,7 +474,20 @@ hm_status Batch::run_subset_moments(const std::vector<int32_t> &which, int mode)
     HM_TRY(stage_copy(ctx_, d_tmp_list_.p, which.data(), which.size() * 4));
     HM_TRY(stage_descriptors());
     HM_TRY(stage_end(ctx_));
-    HIP_TRY(launch_moments(d_bd_.p, d_tmp_tiles_.p, (int)tiles.size(), mode, s));
+    // the epilogue stages each bunch's potential in LDS (one workgroup per
+    // CU, two where both tables fit) when the largest table fits
+    int lds_entries = 0, grid = 0;
+    if (mode == 1) {
+        int64_t nbmax = 0;
+        for (int32_t i : which)
+            if (rb_[(size_t)i].d.flags & F_WAKE) nbmax = std::max<int64_t>(nbmax, rb_[(size_t)i].d.nbins);
+        const int64_t cap = ((int64_t)ctx_->lds_max - 1024) / 8;
+        if (nbmax >= 2) {
+            lds_entries = (int)std::min(nbmax, cap);
+            grid = ctx_->cus * (2 * ((int64_t)lds_entries * 8 + 1024) <= (int64_t)ctx_->lds_max ? 2 : 1);
+        }
+    }
+    HIP_TRY(launch_moments(d_bd_.p, d_tmp_tiles_.p, (int)tiles.size(), mode, grid, lds_entries, s));
     HIP_TRY(launch_moments_final(d_bd_.p, d_tmp_list_.p, (int)which.size(), mode, s));
     for (size_t k = 0; k < which.size(); ++k) rb_[(size_t)which[k]].d.tile_begin = saved[k];
     dirty_ = true;  // descriptors on the device carry the temporary tile_begin

@@ -1625,14 +1625,11 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_sum_deep(
 // wake kick and re-centre (the epilogue) and writes dE back.  The tile is read
 // in the turn pass's layout, 16-byte (z, z) and (dE, dE) pairs, thread t taking
 // pairs lt*ppt + t + k*1024, two pairs' loads in flight; the single particle
-// of an odd bunch goes to thread 0 of the bunch's last tile.
-__global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__restrict__ bds,
-                                                          const int32_t *__restrict__ tile_bunch,
-                                                          int mode) {
-    __shared__ double s_red[4 * 16];
-    const int tile = blockIdx.x;
-    const BunchDev &B = bds[tile_bunch[tile]];
-    const int lt = tile - B.tile_begin;
+// of an odd bunch goes to thread 0 of the bunch's last tile.  LP: the kick
+// reads P from the LDS copy sP (the whole table) instead of global memory.
+template <bool LP>
+__device__ inline void moments_tile(const BunchDev &B, int lt, int mode, const double *sP,
+                                    double *s_red) {
     const int64_t ppt = B.pairs_per_tile;
     const int64_t p0 = (int64_t)lt * ppt;
     const int64_t p1 = min(p0 + ppt, B.npairs);
@@ -1646,7 +1643,9 @@ __global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__rest
     g_f64 *ep = (g_f64 *)B.dE;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     auto one = [&](double z, double e) -> double {
-        if (kick) e = e - interp_pot(c.P, z, c.c0, c.c1, c.inv_s, c.nb);   // wakefield.jl:342-344
+        if (kick)                                                           // wakefield.jl:342-344
+            e = e - (LP ? interp_pot(sP, z, c.c0, c.c1, c.inv_s, c.nb)
+                        : interp_pot(c.P, z, c.c0, c.c1, c.inv_s, c.nb));
         if (rec) e = e - c.m_prev;                                          // evolution.jl:434
         v[0] += z;
         v[1] += z * z;
@@ -1681,6 +1680,35 @@ __global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__rest
     if (threadIdx.x == 0) {
         double *o = B.moments + 4 * (int64_t)lt;
         o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3];
+    }
+}
+
+// The tiles of a table, tile = blockIdx.x + k*gridDim.x.  With dynamic LDS
+// (lds_entries > 0: the epilogue), a workgroup copies a bunch's potential
+// into its LDS once for its tiles of that bunch when the bunch has a kick
+// pending and its table fits; the per-tile arithmetic is the same either way.
+__global__ __launch_bounds__(kPassThreads) void k_moments(const BunchDev *__restrict__ bds,
+                                                          const int32_t *__restrict__ tile_bunch,
+                                                          int ntiles, int mode, int lds_entries) {
+    __shared__ double s_red[4 * 16];
+    extern __shared__ __align__(16) double s_pot[];
+    int staged = -1;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();   // s_red and s_pot of the previous tile are no longer read
+        const int bi = tile_bunch[tile];
+        const BunchDev &B = bds[bi];
+        const int lt = tile - B.tile_begin;
+        const bool lp = mode == 1 && B.sc->pending_kick && B.nbins >= 2 &&
+                        B.nbins <= lds_entries;
+        if (lp && staged != bi) {
+            for (int k = threadIdx.x; k < B.nbins; k += blockDim.x) s_pot[k] = B.P[k];
+            staged = bi;
+            __syncthreads();
+        }
+        if (lp)
+            moments_tile<true>(B, lt, mode, s_pot, s_red);
+        else
+            moments_tile<false>(B, lt, mode, nullptr, s_red);
     }
 }
 
@@ -2155,9 +2183,11 @@ hipError_t launch_stream_copy(const double *src, double *dst, int64_t n, int blo
 }
 
 hipError_t launch_moments(const BunchDev *bds, const int32_t *tb, int ntiles, int mode,
-                          hipStream_t s) {
+                          int grid, int lds_entries, hipStream_t s) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_moments, dim3(ntiles), dim3(kPassThreads), 0, s, bds, tb, mode);
+    grid = grid > 0 ? min(grid, ntiles) : ntiles;
+    hipLaunchKernelGGL(k_moments, dim3(grid), dim3(kPassThreads), (size_t)lds_entries * 8, s, bds,
+                       tb, ntiles, mode, lds_entries);
     return hipGetLastError();
 }
 

@@ -63,7 +63,7 @@ hipError_t configure_kernels(int max_dynamic_lds);
 // per-tile sums of z, z^2, dE, dE^2; mode 1 first applies the pending wake
 // kick and re-centre (the epilogue of the last turn)
 hipError_t launch_moments(const BunchDev *bds, const int32_t *tile_bunch, int ntiles, int mode,
-                          hipStream_t s);
+                          int grid, int lds_entries, hipStream_t s);
 // per-bunch spreads from the tile sums; mode 0 -> out[0..1], mode 1 -> out[2..8]
 hipError_t launch_moments_final(const BunchDev *bds, const int32_t *bunch_list, int nb, int mode,
                                 hipStream_t s);
