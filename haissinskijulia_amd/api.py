@@ -26,8 +26,10 @@ library status code.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import math
+import weakref
 from dataclasses import dataclass, field, fields
 from typing import Optional, Sequence
 
@@ -244,13 +246,18 @@ class EvolveResult:
 # ---------------------------------------------------------------------------
 
 class MI355X:
-    """Device token for the ``comm`` slot: one context (GPU + stream)."""
+    """Device token for the ``comm`` slot: one context (GPU + stream).
+
+    Bunches and servers on the context are released before it (``close``
+    releases them first), and every context still open when the interpreter
+    exits is released by an ``atexit`` hook, while the HIP runtime is whole."""
 
     _default: "MI355X | None" = None
 
     def __init__(self, device: int = 0):
         self.device = device
         self._ctx = None
+        self._dependents = weakref.WeakSet()   # bunches and servers on this context
 
     @property
     def ctx(self):
@@ -259,6 +266,7 @@ class MI355X:
             h = C.c_void_p()
             check(L.hm_create(self.device, C.byref(h)))
             self._ctx = h
+            _open_devices.add(self)
         return self._ctx
 
     def info(self) -> dict:
@@ -270,6 +278,8 @@ class MI355X:
 
     def close(self) -> None:
         if self._ctx is not None:
+            for d in list(self._dependents):   # they live on the context
+                d.close()
             _lib.load().hm_destroy(self._ctx)
             self._ctx = None
 
@@ -282,6 +292,21 @@ class MI355X:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+# contexts with an open hm_ctx; released at interpreter exit, each after its
+# bunches and servers (during module teardown a __del__ may find this module
+# or the runtime already gone)
+_open_devices: "weakref.WeakSet[MI355X]" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_open_devices() -> None:
+    for dev in list(_open_devices):
+        try:
+            dev.close()
         except Exception:
             pass
 
@@ -303,6 +328,7 @@ class DeviceBunch:
         h = C.c_void_p()
         check(_lib.load().hm_bunch_create(self.device.ctx, self.n, C.byref(h)))
         self._h = h
+        self.device._dependents.add(self)
 
     @classmethod
     def from_particles(cls, particles: Particles, device: Optional[MI355X] = None) -> "DeviceBunch":
@@ -430,6 +456,7 @@ class Server:
         check(_lib.load().hm_server_create(self.device.ctx, int(budget_bytes), int(max_slots),
                                            C.byref(h)))
         self._h = h
+        self.device._dependents.add(self)
 
     def submit(self, job: Job) -> int:
         keep: list = []

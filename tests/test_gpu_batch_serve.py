@@ -325,3 +325,15 @@ def test_tile_knobs_keep_bits_alone_and_batched():
         for (E0x, zx, ex), (E0y, zy, ey) in zip(runs[(0, 0)], runs[(tp, 0)]):
             assert E0y == pytest.approx(E0x, rel=1e-14)
             assert normwise(zy, zx) <= COORD_RTOL and normwise(ey, ex) <= COORD_RTOL
+
+
+def test_closing_a_device_releases_its_bunches_and_servers_first():
+    dev = H.MI355X(0)
+    b = H.DeviceBunch(1000, dev)
+    b.generate(3, W.SIGMA_Z, W.SIGMA_E)
+    srv = H.Server(1 << 30, 2, dev)
+    dev.close()
+    assert b._h is None and srv._h is None and dev._ctx is None
+    b.close()
+    srv.close()
+    dev.close()
