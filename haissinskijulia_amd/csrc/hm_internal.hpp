@@ -232,7 +232,7 @@ class Batch {
     std::vector<RunBunch> rb_;
     std::vector<bool> used_;
     DevBuf<BunchDev> d_bd_;
-    DevBuf<int32_t> d_tiles_, d_tmp_tiles_, d_tmp_list_;
+    DevBuf<int32_t> d_tiles_, d_order_, d_tmp_tiles_, d_tmp_list_;
     // per-workgroup tile ranges of the pass and the interpolation sums,
     // [pass_grid_ + 1] then [interp_grid_ + 1] entries, balanced by pairs
     DevBuf<int32_t> d_ranges_;

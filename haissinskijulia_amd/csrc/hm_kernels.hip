@@ -561,6 +561,9 @@ __device__ __forceinline__ void cic_deposit(const PassCtx &c, int cb, double z) 
 // tile lt are lt*ppt + t + k*kPassThreads in ascending k, exactly as for a
 // tile taken alone: the loop keeps its one-pair prefetch running across the
 // tile boundaries and writes each tile's per-wave sums at the tile's end.
+// The run's tiles are run[0..cnt) (indices into the tile table, in ascending
+// order but not necessarily adjacent: a bunch shared by several workgroups
+// has its tiles dealt to them in turn).
 // KM, how the kick reads P: kFullWindow (the LDS window holds the whole table:
 // no global gathers), kGatherFirst or kPrefetchFirst (a partial window, the
 // particles outside it gather from global memory; see the loop body).
@@ -568,7 +571,8 @@ __device__ __forceinline__ void cic_deposit(const PassCtx &c, int cb, double z) 
 // counters at byte cb.
 enum KickMode { kFullWindow = 0, kGatherFirst = 1, kPrefetchFirst = 2 };
 template <uint32_t FX, bool LH, int KM, bool CIC>
-__device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt1, int hb, int cb) {
+__device__ __forceinline__ void turn_pass_run(const BunchDev &B, const int32_t *__restrict__ run,
+                                              int cnt, int hb, int cb) {
     PassCtx c;
     load_ctx(B, c);
     const Flags<FX> fl{c.flags};
@@ -577,7 +581,7 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
     const int64_t n = B.n;
     const int64_t ppt = B.pairs_per_tile;
     const int64_t full = n >> 1;                    // pairs with two particles
-    const int64_t pf_run = min((int64_t)lt1 * ppt, full);
+    const int tb = B.tile_begin;
     const uint32_t tabs = (uint32_t)(B.turn_offset + sc.turn);
     const bool inj = fl.has(F_INJ);
     const bool qe = fl.has(F_QE);
@@ -593,17 +597,6 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
     g_u32 *ghist = (g_u32 *)hist_of(B, sc.turn);
 
     double2 zn = make_double2(0.0, 0.0), en = make_double2(0.0, 0.0);
-    {
-        const int64_t p = (int64_t)lt0 * ppt + threadIdx.x;
-        if (p < pf_run) {
-            zn = ldnt((const g_f64x2 *)(zp + 2 * p));
-            en = ldnt((const g_f64x2 *)(ep + 2 * p));
-        }
-        // the run's first pair completes here (s_waitcnt vmcnt(0)), so that in
-        // the loop the pair in hand is never taken for an outstanding load and
-        // the waits there count only the loads issued in the loop
-        __builtin_amdgcn_s_waitcnt(0x0F70);
-    }
     // The pair stride is the block size, as the compile-time constant: read
     // from the dispatch packet, blockDim.x was reloaded per tile with a vector
     // load and a full wait (1e8 pass +2%).
@@ -618,13 +611,40 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
     };
     // the single particle of an odd bunch sits in the bunch's last tile, the
     // last of this run if the run has it: that tile is flushed after the loop
-    const bool odd_tile = (n & 1) && full >= (int64_t)(lt1 - 1) * ppt &&
-                          full < min((int64_t)lt1 * ppt, B.npairs);
+    const int lt_last = run[cnt - 1] - tb;
+    const bool odd_tile = (n & 1) && full >= (int64_t)lt_last * ppt &&
+                          full < min((int64_t)(lt_last + 1) * ppt, B.npairs);
     double v[4];   // sum z, sum z^2, sum (dE'-dE_init), in-range count
-    for (int lt = lt0; lt < lt1; ++lt) {
+    bool reload = true;   // this tile's first pair was not prefetched
+    for (int t = 0; t < cnt; ++t) {
+        const int lt = run[t] - tb;
         const int64_t p0 = (int64_t)lt * ppt;
         const int64_t p1 = min(p0 + ppt, B.npairs);
         const int64_t pf = min(p1, full);
+        if (reload) {
+            const int64_t p = p0 + threadIdx.x;
+            if (p < pf) {
+                zn = ldnt((const g_f64x2 *)(zp + 2 * p));
+                en = ldnt((const g_f64x2 *)(ep + 2 * p));
+            }
+            // the tile's first pair completes here (s_waitcnt vmcnt(0)), so that
+            // in the loop the pair in hand is never taken for an outstanding
+            // load and the waits there count only the loads issued in the loop
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+        }
+        // where the prefetch goes after this tile's last pair: the next tile's
+        // first pair of this thread when every thread has one there; else this
+        // tile's first pair, read again and not used (and the next tile, the
+        // bunch's short last one, loads its first pair itself)
+        int64_t nxt = p0;
+        reload = false;
+        if (t + 1 < cnt) {
+            const int64_t q0 = (int64_t)(run[t + 1] - tb) * ppt;
+            if (min(q0 + ppt, full) - q0 >= step)
+                nxt = q0;
+            else
+                reload = true;
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] = 0.0;
         for (int64_t p = p0 + threadIdx.x; p < pf; p += step) {
@@ -640,10 +660,10 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             constexpr bool gather_first = KM == kGatherFirst;
             const int64_t pn = p + step;
             // the next pair (the next tile's first at a tile end); past the end
-            // of the run the current pair is read again and not used, so the
+            // of the run a pair of this tile is read again and not used, so the
             // load is unconditional and zn, en need no copy of their old value
             auto prefetch = [&]() {
-                const int64_t pq = pn < pf_run ? pn : p;
+                const int64_t pq = pn < pf ? pn : nxt + threadIdx.x;
                 zn = ldnt((const g_f64x2 *)(zp + 2 * pq));
                 en = ldnt((const g_f64x2 *)(ep + 2 * pq));
             };
@@ -727,13 +747,13 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             stnt((g_f64x2 *)(zp + 2 * p), z0, z1);
             stnt((g_f64x2 *)(ep + 2 * p), e0, e1);
         }
-        if (odd_tile && lt == lt1 - 1) break;   // v kept for the single particle
+        if (odd_tile && t == cnt - 1) break;   // v kept for the single particle
         if (!wake) { v[0] = 0.0; v[1] = 0.0; }
         flush(v, lt);
     }
     // the single particle of an odd bunch (last pair), out of the tile loop so
     // that its inlined turn does not add to the loop's register pressure
-    if (odd_tile && lt0 < lt1) {
+    if (odd_tile) {
         if (threadIdx.x == 0) {
             const int64_t i = n - 1;
             double z0 = zp[i], e0 = ep[i], r0 = 0.0, r1 = 0.0;
@@ -751,7 +771,7 @@ __device__ __forceinline__ void turn_pass_run(const BunchDev &B, int lt0, int lt
             ep[i] = e0;
         }
         if (!wake) { v[0] = 0.0; v[1] = 0.0; }
-        flush(v, lt1 - 1);
+        flush(v, lt_last);
     }
 }
 
@@ -805,32 +825,35 @@ __device__ inline void stage_bunch(const BunchDev &B) {
 }
 
 // The fused pass over a tile table.  Each workgroup takes a contiguous range
-// of tiles with about the same number of pairs as every other workgroup's
-// (one workgroup per CU), stages a bunch's LDS once for its run of
-// tiles in the range, streams the run and flushes the histogram when it moves
-// on.  The per-tile sums depend on the tile and the block size only.
+// of entries of the pass's tile order, with about the same number of pairs as
+// every other workgroup's (one workgroup per CU), stages a bunch's LDS once
+// for its run of the bunch's tiles in the range, streams the run and flushes
+// the histogram when it moves on.  The per-tile sums depend on the tile and
+// the block size only, not on which workgroup takes the tile.
 template <uint32_t FX>
 __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bds,
                                                  const int32_t *__restrict__ tile_bunch,
+                                                 const int32_t *__restrict__ order,
                                                  const int32_t *__restrict__ ranges) {
     // kStdFlags has the wake on and F_GHIST off: its histogram is in LDS
     constexpr bool kLdsOnly = (FX & F_WAKE) && !(FX & F_GHIST);
-    const int t0 = ranges[blockIdx.x];
-    const int t1 = ranges[blockIdx.x + 1];
+    const int e0 = ranges[blockIdx.x];
+    const int e1 = ranges[blockIdx.x + 1];
     int staged = -1;
-    for (int tile = t0; tile < t1;) {
-        const int bi = tile_bunch[tile];
+    for (int e = e0; e < e1;) {
+        const int bi = tile_bunch[order[e]];
         const BunchDev &B = bds[bi];
-        // the bunch's tiles are contiguous in the table: this workgroup's run
-        // of them ends at the bunch's last tile or the end of the range
-        const int te = min(t1, B.tile_begin + B.ntiles);
+        // the run: this workgroup's consecutive entries of the bunch
+        int ee = e + 1;
+        while (ee < e1 && tile_bunch[order[ee]] == bi) ++ee;
         if (bi != staged) {
             if (staged >= 0) flush_hist(bds[staged]);
             stage_bunch(B);
             staged = bi;
         }
         const int hb = pass_hist_bytes(B);
-        const int l0 = tile - B.tile_begin, l1 = te - B.tile_begin;
+        const int32_t *run = order + e;
+        const int cnt = ee - e;
         // a partial window of P: gather first in the pass with compile-time
         // flags (only it has the registers), prefetch first in the generic one
         constexpr int kPartial = FX != 0 ? kGatherFirst : kPrefetchFirst;
@@ -839,37 +862,38 @@ __device__ __forceinline__ void turn_pass_blocks(const BunchDev *__restrict__ bd
         if (kLdsOnly || hb) {
             if (B.flags & F_CIC) {   // the deposit needs the LDS histogram's room
                 if (full)
-                    turn_pass_run<FX, true, kFullWindow, true>(B, l0, l1, hb, cb);
+                    turn_pass_run<FX, true, kFullWindow, true>(B, run, cnt, hb, cb);
                 else
-                    turn_pass_run<FX, true, kPartial, true>(B, l0, l1, hb, cb);
+                    turn_pass_run<FX, true, kPartial, true>(B, run, cnt, hb, cb);
             } else {
                 if (full)
-                    turn_pass_run<FX, true, kFullWindow, false>(B, l0, l1, hb, cb);
+                    turn_pass_run<FX, true, kFullWindow, false>(B, run, cnt, hb, cb);
                 else
-                    turn_pass_run<FX, true, kPartial, false>(B, l0, l1, hb, cb);
+                    turn_pass_run<FX, true, kPartial, false>(B, run, cnt, hb, cb);
             }
         } else {
             if (full)
-                turn_pass_run<FX, false, kFullWindow, false>(B, l0, l1, hb, cb);
+                turn_pass_run<FX, false, kFullWindow, false>(B, run, cnt, hb, cb);
             else
-                turn_pass_run<FX, false, kPrefetchFirst, false>(B, l0, l1, hb, cb);
+                turn_pass_run<FX, false, kPrefetchFirst, false>(B, run, cnt, hb, cb);
         }
-        tile = te;
+        e = ee;
     }
     if (staged >= 0) flush_hist(bds[staged]);
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_turn_pass(const BunchDev *__restrict__ bds,
                                                             const int32_t *__restrict__ tile_bunch,
+                                                            const int32_t *__restrict__ order,
                                                             const int32_t *__restrict__ ranges) {
-    turn_pass_blocks<0>(bds, tile_bunch, ranges);
+    turn_pass_blocks<0>(bds, tile_bunch, order, ranges);
 }
 
 // the same for tables whose bunches all have exactly kStdFlags
 __global__ __launch_bounds__(kPassThreads) void k_turn_pass_std(
     const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
-    const int32_t *__restrict__ ranges) {
-    turn_pass_blocks<kStdFlags>(bds, tile_bunch, ranges);
+    const int32_t *__restrict__ order, const int32_t *__restrict__ ranges) {
+    turn_pass_blocks<kStdFlags>(bds, tile_bunch, order, ranges);
 }
 
 // ---------------------------------------------------------------------------
@@ -2204,15 +2228,15 @@ hipError_t launch_init(const BunchDev *bds, const int32_t *list, int nb, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, const int32_t *ranges, int grid,
-                            size_t lds, int std_only, hipStream_t s) {
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tb, const int32_t *order,
+                            const int32_t *ranges, int grid, size_t lds, int std_only, hipStream_t s) {
     if (grid <= 0) return hipSuccess;
     if (std_only)
         hipLaunchKernelGGL(k_turn_pass_std, dim3((unsigned)grid), dim3(kPassThreads), lds, s, bds,
-                           tb, ranges);
+                           tb, order, ranges);
     else
         hipLaunchKernelGGL(k_turn_pass, dim3((unsigned)grid), dim3(kPassThreads), lds, s, bds, tb,
-                           ranges);
+                           order, ranges);
     return hipGetLastError();
 }
 

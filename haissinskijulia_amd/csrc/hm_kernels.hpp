@@ -71,8 +71,8 @@ hipError_t launch_init(const BunchDev *bds, const int32_t *bunch_list, int nb, h
 // `grid` workgroups over the tile table, workgroup g taking tiles
 // [ranges[g], ranges[g+1]); lds_bytes = the largest per-bunch histogram +
 // potential window; std_only: every tile's bunch has (flags & kPhysMask) == kStdFlags
-hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *ranges,
-                            int grid, size_t lds_bytes, int std_only, hipStream_t s);
+hipError_t launch_turn_pass(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *order,
+                            const int32_t *ranges, int grid, size_t lds, int std_only, hipStream_t s);
 // items are (bunch, block) pairs, for CONV_SPEC (bunch, frequency chunk)
 // pairs; Fmax is the largest FFT length among them, for CONV_SPEC the largest
 // conv_spec_lds of their bunches in bytes
