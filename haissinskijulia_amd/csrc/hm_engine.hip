@@ -295,12 +295,14 @@ static int64_t pairs_per_tile_for(int64_t npairs, int threads, int32_t tile_pair
 // streaming its own contiguous block of two read and two written FP64 arrays
 // sustain 3.8 TB/s, the same blocks dealt as 8192-pair tiles 5.2 TB/s.  Each
 // workgroup's entries stay in ascending order.
-static std::vector<int32_t> deal_tiles(const std::vector<int32_t> &tiles,
-                                       const std::vector<int32_t> &ranges, int grid) {
+static void deal_tiles(const std::vector<int32_t> &tiles, const int32_t *ranges, int grid,
+                       std::vector<int32_t> &out) {
     const int nt = (int)tiles.size();
-    std::vector<int32_t> order((size_t)nt);
-    for (int t = 0; t < nt; ++t) order[(size_t)t] = t;
-    if (grid <= 1) return order;
+    const size_t base = out.size();
+    out.resize(base + (size_t)nt);
+    int32_t *order = out.data() + base;
+    for (int t = 0; t < nt; ++t) order[t] = t;
+    if (grid <= 1) return;
     std::vector<int> ws;
     std::vector<int32_t> start, need, fill;
     for (int b0 = 0; b0 < nt;) {   // the bunch's tiles: [b0, b1) of the table
@@ -320,11 +322,10 @@ static std::vector<int32_t> deal_tiles(const std::vector<int32_t> &tiles,
             int t = b0;
             while (t < b1)
                 for (size_t k = 0; k < ws.size() && t < b1; ++k)
-                    if (fill[k] < need[k]) order[(size_t)(start[k] + fill[k]++)] = t++;
+                    if (fill[k] < need[k]) order[start[k] + fill[k]++] = t++;
         }
         b0 = b1;
     }
-    return order;
 }
 
 // Splits tiles with weights w (pairs) into `grid` contiguous ranges of
@@ -467,13 +468,17 @@ hm_status Batch::build_tables() {
     pass_grid_ = std::min(ntiles_, ctx_->cus);
     interp_grid_ = any_wake_e0_ ? std::min(ntiles_, per_cu * ctx_->cus) : 0;
     interp_deep_ = per_cu == 1;
-    std::vector<int32_t> ranges;
+    // ranges: [pass_grid_ + 1] then [interp_grid_ + 1] boundaries; order: the
+    // pass's tile order, then the interpolation sums'
+    std::vector<int32_t> ranges, order;
     balanced_ranges(wpass, pass_grid_, ranges);
-    std::vector<int32_t> order = deal_tiles(tiles, ranges, pass_grid_);
     balanced_ranges(winterp, interp_grid_, ranges);
+    deal_tiles(tiles, ranges.data(), pass_grid_, order);
+    deal_tiles(tiles, ranges.data() + pass_grid_ + 1, interp_grid_, order);
     // one staging round for every table and the descriptors
     size_t bytes = stage_size(ranges.size() * 4) + stage_size(list.size() * 4) +
-                   stage_size(red.size() * 4) + 2 * stage_size(tiles.size() * 4) +
+                   stage_size(red.size() * 4) + stage_size(tiles.size() * 4) +
+                   stage_size(order.size() * 4) +
                    stage_size(rb_.size() * sizeof(BunchDev));
     for (int k = 0; k < kConvKinds; ++k) bytes += stage_size(conv[k].size() * 4);
     HM_TRY(d_ranges_.ensure(std::max<size_t>(ranges.size(), 1)));
@@ -901,7 +906,8 @@ hm_status Batch::one_turn() {
     }
     if (timing_) HIP_TRY(hipEventRecord(e2, s));
     if (any_wake_e0_)   // two workgroups per CU when both windows fit the LDS
-        HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, d_ranges_.p + pass_grid_ + 1, interp_grid_,
+        HIP_TRY(launch_interp_sum(d_bd_.p, d_tiles_.p, d_order_.p + ntiles_, d_ranges_.p + pass_grid_ + 1,
+                                  interp_grid_,
                                   interp_lds_, interp_deep_, s));
     HIP_TRY(whole ? launch_finalize_whole(d_bd_.p, d_list_.p, nlist_, s)
                   : launch_finalize(d_bd_.p, d_red_.p, nred_, s));

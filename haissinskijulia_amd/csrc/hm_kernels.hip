@@ -1387,7 +1387,8 @@ __global__ __launch_bounds__(kConvThreads) void k_fft_wake(const BunchDev *__res
 // this group's sums; the group loop has the same trip count in every lane.
 // A tile's sums therefore depend on the tile and the block size only (not on G).
 template <int G, bool FULLW>
-__device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0, int lt1) {
+__device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, const int32_t *__restrict__ run,
+                                                     int nrun) {
     extern __shared__ double sP[];
     const int nb = B.nbins;
     const double *__restrict__ P = B.P;
@@ -1410,9 +1411,8 @@ __device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0,
     // reads pair 0 and ignores it), so that every wait for a slot is the same
     // count of younger loads.  Needs a two-particle pair: full >= 1.
     const bool ring = nb >= 2 && full >= 1;
-    const int nrun = lt1 - lt0;
     // the j-th tile of the run, and its two-particle pairs
-    auto tile_of = [&](int j) { return lt0 + j; };
+    auto tile_of = [&](int j) { return run[j] - B.tile_begin; };
     auto lim_of = [&](int lt) {
         const int64_t p0 = (int64_t)lt * ppt;
         return (int)(min(min(p0 + ppt, B.npairs), full) - p0);
@@ -1492,7 +1492,8 @@ __device__ __forceinline__ void interp_sum_run_tiles(const BunchDev &B, int lt0,
 // pipeline start.  A tile's sum is written when its last pair is added; it
 // depends on the tile and the block size only.
 template <bool FULLW>
-__device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, int lt1) {
+__device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, const int32_t *__restrict__ run,
+                                                    int nrun) {
     extern __shared__ double sP[];
     const int nb = B.nbins;
     const double *__restrict__ P = B.P;
@@ -1525,8 +1526,10 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
         if ((threadIdx.x & 63) == 63) B.ipart[(int64_t)lt * kTileWaves + (threadIdx.x >> 6)] = s1[0];
         v = 0.0;
     };
+    auto tile_of = [&](int j) { return run[j] - B.tile_begin; };
     if (nb < 2 || full < 1) {   // a one-bin table, or no two-particle pair
-        for (int lt = lt0; lt < lt1; ++lt) {
+        for (int j = 0; j < nrun; ++j) {
+            const int lt = tile_of(j);
             double v = 0.0;
             const int lim = lim_of(lt);
             if (nb < 2)
@@ -1538,22 +1541,23 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
     // The ring's loads are issued unconditionally (a lane without a next pair
     // reads pair 0 and ignores it), so that every wait for a slot is the same
     // count of younger loads.  The load and compute cursors are uniform.
-    int llt = lt0, lk = 0;
+    int lj = 0, lk = 0;   // the load cursor: entry lj of the run, pair lk of the thread
     auto load_next = [&]() -> double2 {
         const int r = tid + lk * ibs;
-        const bool ok = llt < lt1 && r < lim_of(llt);
+        const int llt = lj < nrun ? tile_of(lj) : 0;
+        const bool ok = lj < nrun && r < lim_of(llt);
         const double2 x = ldnt(ok ? zpair + (int64_t)llt * ppt + r : zpair);
         if (++lk == K) {
             lk = 0;
-            ++llt;
+            ++lj;
         }
         return x;
     };
     double2 zn[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) zn[u] = load_next();
-    const int J = (lt1 - lt0) * K;
-    int clt = lt0, ck = 0, clim = lim_of(lt0);
+    const int J = nrun * K;
+    int cj = 0, clt = tile_of(0), ck = 0, clim = lim_of(clt);
     double v = 0.0;
     for (int j = 0; j < J; j += 4) {
 #pragma unroll
@@ -1579,8 +1583,10 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
             if (++ck == K) {   // the last pair of tile clt
                 flush(v, clt);
                 ck = 0;
-                ++clt;
-                if (clt < lt1) clim = lim_of(clt);
+                if (++cj < nrun) {
+                    clt = tile_of(cj);
+                    clim = lim_of(clt);
+                }
             }
         }
     }
@@ -1594,37 +1600,42 @@ __device__ __forceinline__ void interp_sum_run_flat(const BunchDev &B, int lt0, 
 // FULLW: the window holds the whole table (no gathers), so the compiler's
 // waits for the ring's slots count only the ring's own loads.
 template <int G, bool FULLW>
-__device__ __forceinline__ void interp_sum_run_w(const BunchDev &B, int lt0, int lt1) {
+__device__ __forceinline__ void interp_sum_run_w(const BunchDev &B, const int32_t *run, int nrun) {
     if (G > 4 && B.pairs_per_tile >= G * (int64_t)kInterpThreads)
-        interp_sum_run_tiles<G, FULLW>(B, lt0, lt1);
+        interp_sum_run_tiles<G, FULLW>(B, run, nrun);
     else if (B.pairs_per_tile >= 4 * (int64_t)kInterpThreads)
-        interp_sum_run_tiles<4, FULLW>(B, lt0, lt1);
+        interp_sum_run_tiles<4, FULLW>(B, run, nrun);
     else
-        interp_sum_run_flat<FULLW>(B, lt0, lt1);
+        interp_sum_run_flat<FULLW>(B, run, nrun);
 }
 
 template <int G>
-__device__ __forceinline__ void interp_sum_run(const BunchDev &B, int lt0, int lt1) {
+__device__ __forceinline__ void interp_sum_run(const BunchDev &B, const int32_t *run, int nrun) {
     if (B.iwin >= B.nbins)
-        interp_sum_run_w<G, true>(B, lt0, lt1);
+        interp_sum_run_w<G, true>(B, run, nrun);
     else
-        interp_sum_run_w<G, false>(B, lt0, lt1);
+        interp_sum_run_w<G, false>(B, run, nrun);
 }
 
 // The interpolation sums of the wake + update_E0 bunches over a tile table:
-// each workgroup walks its contiguous range of tiles, one run per bunch.
+// each workgroup walks its contiguous range of entries of the sums' tile
+// order, one run per bunch (a bunch shared by several workgroups has its
+// tiles dealt to them in turn, as in the pass).
 template <int G>
 __device__ __forceinline__ void interp_sum_blocks(const BunchDev *__restrict__ bds,
                                          const int32_t *__restrict__ tile_bunch,
+                                         const int32_t *__restrict__ order,
                                          const int32_t *__restrict__ ranges) {
-    const int t0 = ranges[blockIdx.x];
-    const int t1 = ranges[blockIdx.x + 1];
-    for (int tile = t0; tile < t1;) {
-        const BunchDev &B = bds[tile_bunch[tile]];
-        const int te = min(t1, B.tile_begin + B.ntiles);
+    const int e0 = ranges[blockIdx.x];
+    const int e1 = ranges[blockIdx.x + 1];
+    for (int e = e0; e < e1;) {
+        const int bi = tile_bunch[order[e]];
+        const BunchDev &B = bds[bi];
+        int ee = e + 1;
+        while (ee < e1 && tile_bunch[order[ee]] == bi) ++ee;
         if ((B.flags & F_WAKE) && (B.flags & F_E0) && !(B.flags & F_CIC))
-            interp_sum_run<G>(B, tile - B.tile_begin, te - B.tile_begin);
-        tile = te;
+            interp_sum_run<G>(B, order + e, ee - e);
+        e = ee;
     }
 }
 
@@ -1632,16 +1643,16 @@ __device__ __forceinline__ void interp_sum_blocks(const BunchDev *__restrict__ b
 // LDS; rings of four loads.
 __global__ __launch_bounds__(kInterpThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_interp_sum(
     const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
-    const int32_t *__restrict__ ranges) {
-    interp_sum_blocks<4>(bds, tile_bunch, ranges);
+    const int32_t *__restrict__ order, const int32_t *__restrict__ ranges) {
+    interp_sum_blocks<4>(bds, tile_bunch, order, ranges);
 }
 
 // One workgroup per CU (a window of P larger than half the LDS): the
 // registers of four waves per SIMD allow rings of eight loads.
 __global__ __launch_bounds__(kInterpThreads) void k_interp_sum_deep(
     const BunchDev *__restrict__ bds, const int32_t *__restrict__ tile_bunch,
-    const int32_t *__restrict__ ranges) {
-    interp_sum_blocks<8>(bds, tile_bunch, ranges);
+    const int32_t *__restrict__ order, const int32_t *__restrict__ ranges) {
+    interp_sum_blocks<8>(bds, tile_bunch, order, ranges);
 }
 
 // Per-tile sums of z, z^2, dE, dE^2 (the initial and final spreads,
@@ -2258,15 +2269,16 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
     return hipGetLastError();
 }
 
-hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, const int32_t *ranges,
-                             int grid, size_t lds_bytes, bool deep, hipStream_t s) {
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tb, const int32_t *order,
+                             const int32_t *ranges, int grid, size_t lds_bytes, bool deep,
+                             hipStream_t s) {
     if (grid <= 0) return hipSuccess;
     if (deep)
         hipLaunchKernelGGL(k_interp_sum_deep, dim3((unsigned)grid), dim3(kInterpThreads), lds_bytes,
-                           s, bds, tb, ranges);
+                           s, bds, tb, order, ranges);
     else
         hipLaunchKernelGGL(k_interp_sum, dim3((unsigned)grid), dim3(kInterpThreads), lds_bytes, s,
-                           bds, tb, ranges);
+                           bds, tb, order, ranges);
     return hipGetLastError();
 }
 

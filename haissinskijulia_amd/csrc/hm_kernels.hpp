@@ -81,8 +81,9 @@ hipError_t launch_conv(int kind, const BunchDev *bds, const int32_t *items, int 
 // interpolation sums of the wake + update_E0 bunches: `grid` workgroups over
 // the tile table (ranges as for the pass); lds_bytes = 8 x the largest iwin;
 // deep: one workgroup per CU, eight streaming loads in flight per thread
-hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *ranges,
-                             int grid, size_t lds_bytes, bool deep, hipStream_t s);
+hipError_t launch_interp_sum(const BunchDev *bds, const int32_t *tile_bunch, const int32_t *order,
+                             const int32_t *ranges, int grid, size_t lds_bytes, bool deep,
+                             hipStream_t s);
 // per-bunch reduction of the pass's tile sums into the turn scalars (sum of
 // dE'-dE_init; sigma_z, current, binned count, centroid cell of a wake bunch),
 // and of the interpolation sums into the E0 update.  items holds nitems
