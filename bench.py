@@ -346,8 +346,9 @@ def main() -> int:
                     help="fresh-copy runs at P = 1 (median); 0 skips them")
     ap.add_argument("--cpu-turns-p1", type=int, default=1, help="turns of each P = 1 run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--scan-turns", type=int, default=200,
-                    help="turns of the 256-bunch scan (batched vs plain); 0 skips it")
+    ap.add_argument("--scan-turns", type=int, default=5000,
+                    help="turns of the 256-bunch scan (batched vs plain, about 80 s at 5000); "
+                         "0 skips it")
     ap.add_argument("--scan-batched-only", action="store_true",
                     help="time the scan's batched path only (no plain runs)")
     ap.add_argument("--no-main", action="store_true",
