@@ -325,39 +325,93 @@ __device__ __forceinline__ int bitrev(int m, int logF) {
     return (int)(__brev((unsigned)m) >> (32 - logF));
 }
 
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+    return make_double2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+
 // In-place radix-2 decimation-in-time FFT of X[F] in LDS, bit-reversed input,
-// natural-order output, twiddles tw[k] = exp(-2 pi i k / F).
+// natural-order output, twiddles tw[k] = exp(-2 pi i k / F).  Stage s pairs
+// X[i] and X[i + 2^s] (i with bit s clear, k = i mod 2^s) with twiddle
+// tw[k << (logF-1-s)].  The stages run in pairs (s, s+1) as radix-2^2
+// butterflies: each thread takes the four points base + {0,1,2,3}*2^s
+// through both stages in registers, the same additions and products as the
+// two radix-2 stages (the same bits) with half the barriers and LDS round
+// trips; an odd stage count starts with one radix-2 stage.
 __device__ void fft_dit_forward(double2 *X, int F, int logF, const double2 *__restrict__ tw) {
-    for (int s = 0; s < logF; ++s) {
-        const int h = 1 << s, tsh = logF - 1 - s;
+    int s = 0;
+    if (logF & 1) {   // stage 0 alone: pairs (2b, 2b+1), twiddle tw[0]
         __syncthreads();
         for (int b = threadIdx.x; b < (F >> 1); b += blockDim.x) {
-            const int k = b & (h - 1);
-            const int i = ((b >> s) << (s + 1)) | k;
-            const double2 w = tw[k << tsh];
-            const double2 a = X[i];
-            const double2 t = cmul(X[i + h], w);
-            X[i] = make_double2(a.x + t.x, a.y + t.y);
-            X[i + h] = make_double2(a.x - t.x, a.y - t.y);
+            const double2 a = X[2 * b];
+            const double2 t = cmul(X[2 * b + 1], tw[0]);
+            X[2 * b] = cadd(a, t);
+            X[2 * b + 1] = csub(a, t);
+        }
+        s = 1;
+    }
+    for (; s + 1 < logF; s += 2) {
+        const int h = 1 << s, tsh = logF - 1 - s;
+        __syncthreads();
+        for (int q = threadIdx.x; q < (F >> 2); q += blockDim.x) {
+            const int k = q & (h - 1);
+            const int i0 = ((q >> s) << (s + 2)) | k;
+            const double2 w1 = tw[k << tsh];              // stage s, both pairs
+            const double2 wa = tw[k << (tsh - 1)];        // stage s+1, pair (0, 2)
+            const double2 wb = tw[(k + h) << (tsh - 1)];  // stage s+1, pair (1, 3)
+            const double2 x0 = X[i0], x1 = X[i0 + h], x2 = X[i0 + 2 * h], x3 = X[i0 + 3 * h];
+            double2 t = cmul(x1, w1);
+            const double2 a0 = cadd(x0, t), a1 = csub(x0, t);
+            t = cmul(x3, w1);
+            const double2 a2 = cadd(x2, t), a3 = csub(x2, t);
+            t = cmul(a2, wa);
+            X[i0] = cadd(a0, t);
+            X[i0 + 2 * h] = csub(a0, t);
+            t = cmul(a3, wb);
+            X[i0 + h] = cadd(a1, t);
+            X[i0 + 3 * h] = csub(a1, t);
         }
     }
     __syncthreads();
 }
 
 // Unnormalised inverse FFT (decimation in frequency, conjugate twiddles):
-// natural-order input, output y[m] at X[bitrev(m)].
+// natural-order input, output y[m] at X[bitrev(m)].  Stage s (from logF-1
+// down to 0) takes a = X[i], c = X[i + 2^s] to a + c and (a - c) conj(w),
+// w = tw[k << (logF-1-s)]; stages run in pairs (s, s-1) as radix-2^2
+// butterflies on base + {0, 2^(s-1), 2^s, 2^s + 2^(s-1)}, with the same
+// operations as the radix-2 stages, and an odd stage count ends with stage 0
+// alone.
 __device__ void fft_dif_inverse(double2 *X, int F, int logF, const double2 *__restrict__ tw) {
-    for (int s = logF - 1; s >= 0; --s) {
-        const int h = 1 << s, tsh = logF - 1 - s;
+    int s = logF - 1;
+    for (; s >= 1; s -= 2) {
+        const int h = 1 << s, h2 = h >> 1, tsh = logF - 1 - s;
         __syncthreads();
+        for (int q = threadIdx.x; q < (F >> 2); q += blockDim.x) {
+            const int k = q & (h2 - 1);
+            const int i0 = ((q >> (s - 1)) << (s + 1)) | k;
+            const double2 wa = conj2(tw[k << tsh]);          // stage s, pair (0, 2)
+            const double2 wb = conj2(tw[(k + h2) << tsh]);   // stage s, pair (1, 3)
+            const double2 wc = conj2(tw[k << (tsh + 1)]);    // stage s-1, both pairs
+            const double2 x0 = X[i0], x1 = X[i0 + h2], x2 = X[i0 + h], x3 = X[i0 + h + h2];
+            const double2 a0 = cadd(x0, x2), a2 = cmul(csub(x0, x2), wa);
+            const double2 a1 = cadd(x1, x3), a3 = cmul(csub(x1, x3), wb);
+            X[i0] = cadd(a0, a1);
+            X[i0 + h2] = cmul(csub(a0, a1), wc);
+            X[i0 + h] = cadd(a2, a3);
+            X[i0 + h + h2] = cmul(csub(a2, a3), wc);
+        }
+    }
+    if (s == 0) {   // stage 0 alone: pairs (2b, 2b+1), twiddle conj(tw[0])
+        __syncthreads();
+        const double2 w = conj2(tw[0]);
         for (int b = threadIdx.x; b < (F >> 1); b += blockDim.x) {
-            const int k = b & (h - 1);
-            const int i = ((b >> s) << (s + 1)) | k;
-            const double2 t = tw[k << tsh];
-            const double2 w = make_double2(t.x, -t.y);
-            const double2 a = X[i], c = X[i + h];
-            X[i] = make_double2(a.x + c.x, a.y + c.y);
-            X[i + h] = cmul(make_double2(a.x - c.x, a.y - c.y), w);
+            const double2 a = X[2 * b], c = X[2 * b + 1];
+            X[2 * b] = cadd(a, c);
+            X[2 * b + 1] = cmul(csub(a, c), w);
         }
     }
     __syncthreads();
