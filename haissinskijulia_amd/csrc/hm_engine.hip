@@ -352,7 +352,7 @@ Batch::~Batch() {
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     d_bd_.release(); d_tiles_.release(); d_order_.release();
     for (auto &c : d_conv_) c.release();
-    d_red_.release(); d_list_.release();
+    d_red_.release(); d_list_.release(); d_bw_.release();
     d_tmp_tiles_.release(); d_tmp_list_.release(); d_ranges_.release();
 }
 
@@ -379,8 +379,9 @@ hm_status Batch::stage_descriptors() {
 
 // launch tables of the active bunches
 hm_status Batch::build_tables() {
-    std::vector<int32_t> tiles, conv[kConvKinds], red, list;
+    std::vector<int32_t> tiles, conv[kConvKinds], red, list, bw;
     pass_lds_ = 0;
+    bw_lds_ = 0;
     spec_lds_ = 0;
     Fmax_ = 2;
     any_wake_ = any_wake_e0_ = any_reduce_ = false;
@@ -403,12 +404,28 @@ hm_status Batch::build_tables() {
         if (!used_[i] || !q.active) continue;
         q.d.tile_begin = (int32_t)tiles.size();
         for (int t = 0; t < q.d.ntiles; ++t) tiles.push_back((int32_t)i);
+        if ((q.d.flags & kPhysMask) != kStdFlags) std_only_ = false;
+        if (q.d.flags & F_WAKE) pass_lds_ = std::max(pass_lds_, (size_t)pass_lds_bytes(q.d));
+        // the whole bin phase in one workgroup (k_bin_whole) for a small bunch
+        // without interpolation sums; the other bunches take a launch per phase
+        {
+            const uint32_t f = q.d.flags;
+            const bool wake = (f & F_WAKE) != 0;
+            const bool dep = wake && (f & F_E0) && (f & F_CIC);
+            const size_t lds =
+                bin_whole_layout(wake ? q.d.nbins : 0, q.d.nA, q.d.nJ, q.d.F, dep).bytes;
+            if (q.d.ntiles <= kBinWholeMaxTiles && lds <= kBinWholeMaxLds &&
+                (!wake || !(f & F_E0) || (f & F_CIC))) {
+                bw.push_back((int32_t)i);
+                bw_lds_ = std::max(bw_lds_, lds);
+                continue;
+            }
+        }
         list.push_back((int32_t)i);
         for (int r = 0; r < q.d.nred; ++r) {
             red.push_back((int32_t)i);
             red.push_back(r);
         }
-        if ((q.d.flags & kPhysMask) != kStdFlags) std_only_ = false;
         if (q.d.flags & (F_WAKE | F_E0)) any_reduce_ = true;
         if (q.d.flags & F_WAKE) {
             any_wake_ = true;
@@ -441,12 +458,12 @@ hm_status Batch::build_tables() {
                 spec_lds_ = std::max(spec_lds_, conv_spec_lds(q.d.nA, q.d.nJ, q.d.F));
             }
             Fmax_ = std::max(Fmax_, q.d.F);
-            pass_lds_ = std::max(pass_lds_, (size_t)pass_lds_bytes(q.d));
         }
     }
     ntiles_ = (int)tiles.size();
     nred_ = (int)red.size() / 2;
     nlist_ = (int)list.size();
+    nbw_ = (int)bw.size();
     // Workgroup ranges: contiguous runs of the tile table with equal numbers
     // of pairs (tiles of different bunches differ in size), for the pass and
     // for the interpolation sums (which only count wake + update_E0 bunches).
@@ -477,6 +494,7 @@ hm_status Batch::build_tables() {
     deal_tiles(tiles, ranges.data() + pass_grid_ + 1, interp_grid_, order);
     // one staging round for every table and the descriptors
     size_t bytes = stage_size(ranges.size() * 4) + stage_size(list.size() * 4) +
+                   stage_size(bw.size() * 4) +
                    stage_size(red.size() * 4) + stage_size(tiles.size() * 4) +
                    stage_size(order.size() * 4) +
                    stage_size(rb_.size() * sizeof(BunchDev));
@@ -486,6 +504,7 @@ hm_status Batch::build_tables() {
     HM_TRY(d_order_.ensure(std::max<size_t>(order.size(), 1)));
     HM_TRY(d_red_.ensure(std::max<size_t>(red.size(), 2)));
     HM_TRY(d_list_.ensure(std::max<size_t>(list.size(), 1)));
+    HM_TRY(d_bw_.ensure(std::max<size_t>(bw.size(), 1)));
     for (int k = 0; k < kConvKinds; ++k) {
         nconv_[k] = (int)conv[k].size() / 2;
         HM_TRY(d_conv_[k].ensure(std::max<size_t>(conv[k].size(), 2)));
@@ -493,6 +512,7 @@ hm_status Batch::build_tables() {
     HM_TRY(stage_begin(ctx_, bytes));
     HM_TRY(stage_copy(ctx_, d_ranges_.p, ranges.data(), ranges.size() * 4));
     HM_TRY(stage_copy(ctx_, d_list_.p, list.data(), list.size() * 4));
+    HM_TRY(stage_copy(ctx_, d_bw_.p, bw.data(), bw.size() * 4));
     HM_TRY(stage_copy(ctx_, d_red_.p, red.data(), red.size() * 4));
     for (int k = 0; k < kConvKinds; ++k)
         HM_TRY(stage_copy(ctx_, d_conv_[k].p, conv[k].data(), conv[k].size() * 4));
@@ -892,6 +912,7 @@ hm_status Batch::one_turn() {
     HIP_TRY(launch_turn_pass(d_bd_.p, d_tiles_.p, d_order_.p, d_ranges_.p, pass_grid_, pass_lds_,
                              std_only_ ? 1 : 0, s));
     if (timing_) HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(launch_bin_whole(d_bd_.p, d_bw_.p, nbw_, bw_lds_, s));
     const bool whole = nlist_ >= kWholeMinBunches;
     if (any_reduce_)
         HIP_TRY(whole ? launch_bin_reduce_whole(d_bd_.p, d_list_.p, nlist_, s)
@@ -923,7 +944,7 @@ hm_status Batch::flush_timing() {
         HIP_TRY(hipEventElapsedTime(&b, ev_pool_[k + 1], ev_pool_[k + 2]));
         pass_ms_ += a;
         ++pass_count_;
-        if (any_wake_) {
+        if (any_wake_ || nbw_ > 0) {
             conv_ms_ += b;
             ++conv_count_;
         }

@@ -240,8 +240,13 @@ class Batch {
     bool interp_deep_ = false;   // one interpolation workgroup per CU: deeper rings
     DevBuf<int32_t> d_conv_[kConvKinds];   // convolution items of each ConvKind
     DevBuf<int32_t> d_red_;       // (bunch, slice) items of the reductions
-    DevBuf<int32_t> d_list_;      // the active bunches
-    int ntiles_ = 0, nred_ = 0, nlist_ = 0;
+    DevBuf<int32_t> d_list_;      // the active bunches of the launch-per-phase bin phase
+    DevBuf<int32_t> d_bw_;        // the active bunches of k_bin_whole
+    int ntiles_ = 0, nred_ = 0, nlist_ = 0, nbw_ = 0;
+    size_t bw_lds_ = 0;           // dynamic LDS of k_bin_whole
+    // k_bin_whole takes a bunch whose tables fit this much LDS (up to 1024
+    // bins at the tuned blocks)
+    static constexpr size_t kBinWholeMaxLds = 144 * 1024;
     // one workgroup per bunch for the reductions once the bunches alone fill
     // the device (no cross-workgroup counts, whose device-scope fences cost
     // more than the slices gain there)

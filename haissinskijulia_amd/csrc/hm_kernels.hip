@@ -340,12 +340,14 @@ __device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -
 // butterflies: each thread takes the four points base + {0,1,2,3}*2^s
 // through both stages in registers, the same additions and products as the
 // two radix-2 stages (the same bits) with half the barriers and LDS round
-// trips; an odd stage count starts with one radix-2 stage.
-__device__ void fft_dit_forward(double2 *X, int F, int logF, const double2 *__restrict__ tw) {
+// trips; an odd stage count starts with one radix-2 stage.  nblk blocks of F
+// points, X + b*F, are transformed side by side.
+__device__ void fft_dit_forward(double2 *X, int F, int logF, const double2 *__restrict__ tw,
+                                int nblk = 1) {
     int s = 0;
     if (logF & 1) {   // stage 0 alone: pairs (2b, 2b+1), twiddle tw[0]
         __syncthreads();
-        for (int b = threadIdx.x; b < (F >> 1); b += blockDim.x) {
+        for (int b = threadIdx.x; b < nblk * (F >> 1); b += blockDim.x) {
             const double2 a = X[2 * b];
             const double2 t = cmul(X[2 * b + 1], tw[0]);
             X[2 * b] = cadd(a, t);
@@ -356,9 +358,10 @@ __device__ void fft_dit_forward(double2 *X, int F, int logF, const double2 *__re
     for (; s + 1 < logF; s += 2) {
         const int h = 1 << s, tsh = logF - 1 - s;
         __syncthreads();
-        for (int q = threadIdx.x; q < (F >> 2); q += blockDim.x) {
+        for (int u = threadIdx.x; u < nblk * (F >> 2); u += blockDim.x) {
+            const int q = u & ((F >> 2) - 1);
             const int k = q & (h - 1);
-            const int i0 = ((q >> s) << (s + 2)) | k;
+            const int i0 = ((u >> (logF - 2)) << logF) + (((q >> s) << (s + 2)) | k);
             const double2 w1 = tw[k << tsh];              // stage s, both pairs
             const double2 wa = tw[k << (tsh - 1)];        // stage s+1, pair (0, 2)
             const double2 wb = tw[(k + h) << (tsh - 1)];  // stage s+1, pair (1, 3)
@@ -384,15 +387,17 @@ __device__ void fft_dit_forward(double2 *X, int F, int logF, const double2 *__re
 // w = tw[k << (logF-1-s)]; stages run in pairs (s, s-1) as radix-2^2
 // butterflies on base + {0, 2^(s-1), 2^s, 2^s + 2^(s-1)}, with the same
 // operations as the radix-2 stages, and an odd stage count ends with stage 0
-// alone.
-__device__ void fft_dif_inverse(double2 *X, int F, int logF, const double2 *__restrict__ tw) {
+// alone.  nblk blocks side by side, as in fft_dit_forward.
+__device__ void fft_dif_inverse(double2 *X, int F, int logF, const double2 *__restrict__ tw,
+                                int nblk = 1) {
     int s = logF - 1;
     for (; s >= 1; s -= 2) {
         const int h = 1 << s, h2 = h >> 1, tsh = logF - 1 - s;
         __syncthreads();
-        for (int q = threadIdx.x; q < (F >> 2); q += blockDim.x) {
+        for (int u = threadIdx.x; u < nblk * (F >> 2); u += blockDim.x) {
+            const int q = u & ((F >> 2) - 1);
             const int k = q & (h2 - 1);
-            const int i0 = ((q >> (s - 1)) << (s + 1)) | k;
+            const int i0 = ((u >> (logF - 2)) << logF) + (((q >> (s - 1)) << (s + 1)) | k);
             const double2 wa = conj2(tw[k << tsh]);          // stage s, pair (0, 2)
             const double2 wb = conj2(tw[(k + h2) << tsh]);   // stage s, pair (1, 3)
             const double2 wc = conj2(tw[k << (tsh + 1)]);    // stage s-1, both pairs
@@ -408,7 +413,7 @@ __device__ void fft_dif_inverse(double2 *X, int F, int logF, const double2 *__re
     if (s == 0) {   // stage 0 alone: pairs (2b, 2b+1), twiddle conj(tw[0])
         __syncthreads();
         const double2 w = conj2(tw[0]);
-        for (int b = threadIdx.x; b < (F >> 1); b += blockDim.x) {
+        for (int b = threadIdx.x; b < nblk * (F >> 1); b += blockDim.x) {
             const double2 a = X[2 * b], c = X[2 * b + 1];
             X[2 * b] = cadd(a, c);
             X[2 * b + 1] = cmul(csub(a, c), w);
@@ -1051,8 +1056,9 @@ __device__ inline bool slices_done(const BunchDev &B, int r, const double (&v)[N
 
 // The turn scalars from the reduced pass sums t = {sum z, sum z^2,
 // sum (dE'-dE_init), binned count}.
-__device__ inline void bin_scalars(const BunchDev &B, const double (&t)[4]) {
-    TurnScalars &sc = *B.sc;
+// bin_scalars_on updates the turn scalars sc in place (the bin phase in one
+// workgroup keeps them in registers).
+__device__ inline void bin_scalars_on(const BunchDev &B, const double (&t)[4], TurnScalars &sc) {
     sc.sum_dE = t[2];
     if (!(B.flags & F_WAKE)) return;
     const double sz = gstd_from_sums(t[0], t[1], (double)B.n);
@@ -1072,6 +1078,10 @@ __device__ inline void bin_scalars(const BunchDev &B, const double (&t)[4]) {
         m[1] = sz;
         m[3] = t[3];
     }
+}
+
+__device__ inline void bin_scalars(const BunchDev &B, const double (&t)[4]) {
+    bin_scalars_on(B, t, *B.sc);
 }
 
 // The sum over the nred slices of a per-wave partial array by one 1024-thread
@@ -1855,9 +1865,9 @@ __global__ __launch_bounds__(kFinalThreads) void k_init(const BunchDev *__restri
 // of the next turn.  One workgroup per slice of the interpolation sums.
 // The E0 update (evolution.jl:419-429) from this turn's sums (psum: the sum of
 // the interpolated potential over the bunch), and the scalars of the next turn.
-__device__ inline void finish_turn(const BunchDev &B, double psum) {
+// finish_on updates the turn scalars s in place.
+__device__ inline void finish_on(const BunchDev &B, double psum, TurnScalars &s) {
     const uint32_t fl = B.flags;
-    TurnScalars s = *B.sc;
     double E0 = s.E0;
     if (s.drift_skip) s.drift_skipped_count += 1;
     if (fl & F_E0) {                                  // evolution.jl:419-429
@@ -1872,6 +1882,11 @@ __device__ inline void finish_turn(const BunchDev &B, double psum) {
     if (B.mon) B.mon[4 * s.turn + 2] = E0;   // turn monitor: E0 after the turn
     s.turn += 1;
     turn_scalars(B, E0, s);
+}
+
+__device__ inline void finish_turn(const BunchDev &B, double psum) {
+    TurnScalars s = *B.sc;
+    finish_on(B, psum, s);
     *B.sc = s;
 }
 
@@ -1978,6 +1993,179 @@ __global__ __launch_bounds__(1024) void k_finalize_whole(const BunchDev *__restr
         }
     }
     if (threadIdx.x == 0) finish_turn(B, t[0]);
+}
+
+// cic_psum over the deposit and the potential staged in LDS (sN[nb+1],
+// sH[2 nb], sP[nb]): the same cells per thread in the same order, the same
+// expressions, so the same bits; the global counters are cleared as there.
+__device__ inline double cic_psum_lds(const BunchDev &B, const uint32_t *sN,
+                                      const unsigned long long *sH, const double *sP,
+                                      double *s_red) {
+    const int nb = B.nbins;
+    g_u64 *H = (g_u64 *)B.cic;
+    g_u32 *N = (g_u32 *)(B.cic + 2 * nb);
+    if ((int)threadIdx.x < kRedThreads) {
+        double v = 0.0;
+        for (int k = (int)threadIdx.x; k < nb - 1; k += kRedThreads) {
+            const uint32_t cnt = sN[k];
+            if (cnt) {
+                const double sk = ((double)sH[k] * (double)(1 << kCicLoBits) + (double)sH[nb + k]) *
+                                  (1.0 / kCicScale);
+                v += ((double)cnt - sk) * sP[k] + sk * sP[k + 1];
+                N[k] = 0u;
+                H[k] = 0ull;
+                H[nb + k] = 0ull;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kRedThreads / 64; ++w) t += s_red[w];
+        const uint32_t bad = sN[nb];
+        N[nb] = 0u;
+        if (bad) t = __builtin_nan("");
+    }
+    return t;
+}
+
+// The bin phase of a small bunch in one 1024-thread workgroup, in place of
+// k_bin_reduce, the convolution kernels and k_finalize.  Everything it reads
+// is staged first, side by side (the turn scalars in thread 0's registers,
+// this turn's counts, the line-density kernel, the wake spectra, the
+// deposit), so the phase waits on memory about twice instead of once per
+// step.  Then: the pass sums reduced as whole_sum does (the sliced form's
+// bits) into the turn scalars; the line density of every block packed and
+// transformed side by side (nA blocks of F points); the spectral sums Y_o =
+// sum_j X_{o-j} FFT(W_j) added in ascending j from zero, as k_conv_inv adds
+// them, one (block, frequency) pair per thread; the inverse transforms side
+// by side; P and P2 (and P in LDS); the E0 update from the deposit as
+// k_finalize_whole.  Every value is the same expression of the same operands
+// as on the launch-per-phase path, so the bunch gets the same bits either
+// way.  LDS layout: bin_whole_layout.
+__device__ inline void bin_phase_whole(const BunchDev &B) {
+    extern __shared__ __align__(16) uint8_t s_bw[];
+    __shared__ double s_nh, s_cur, s_red[16];
+    const uint32_t fl = B.flags;
+    const bool wake = (fl & F_WAKE) != 0;
+    const bool cic = wake && (fl & F_E0) && (fl & F_CIC);
+    const int nb = B.nbins, nA = B.nA, F = B.F, logF = B.logF, Bk = B.B, nJ = B.nJ;
+    BinWholeLayout L = bin_whole_layout(wake ? nb : 0, nA, nJ, F, cic);
+    double2 *sX = reinterpret_cast<double2 *>(s_bw);
+    double2 *sW = sX + nA * F, *sY = sW + nJ * F;
+    uint32_t *sHist = reinterpret_cast<uint32_t *>(s_bw + L.hist);
+    double *sLam = reinterpret_cast<double *>(s_bw + L.lam);
+    double *sP = reinterpret_cast<double *>(s_bw + L.pot);
+    unsigned long long *sH = reinterpret_cast<unsigned long long *>(s_bw + L.cic);
+    uint32_t *sN = reinterpret_cast<uint32_t *>(sH + 2 * nb);
+    TurnScalars sc;
+    if (threadIdx.x == 0) sc = *B.sc;
+    const int64_t turn = B.sc->turn;
+    if (wake) {
+        const uint32_t *h = hist_of(B, turn);
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            sHist[i] = h[i];
+            sLam[i] = B.lam[i];
+        }
+        if (nJ > 1) {
+            const double2 *What = reinterpret_cast<const double2 *>(B.What);
+            for (int u = threadIdx.x; u < nJ * F; u += blockDim.x) sW[u] = What[u];
+        }
+        if (cic) {
+            const unsigned long long *C = B.cic;
+            const uint32_t *N = reinterpret_cast<const uint32_t *>(B.cic + 2 * nb);
+            for (int i = threadIdx.x; i < 2 * nb; i += blockDim.x) sH[i] = C[i];
+            for (int i = threadIdx.x; i <= nb; i += blockDim.x) sN[i] = N[i];
+        }
+    }
+    if (fl & (F_WAKE | F_E0)) {
+        double t[4];
+        whole_sum<4>(B, B.part, t);   // its barriers also publish the staged LDS
+        if (threadIdx.x == 0) {
+            bin_scalars_on(B, t, sc);
+            s_nh = sc.n_h;
+            s_cur = sc.current;
+        }
+    }
+    __syncthreads();
+    if (wake) {
+        const double nh = s_nh, cur = s_cur;
+        const double2 *tw = reinterpret_cast<const double2 *>(B.tw);
+        uint32_t *hn = hist_of(B, turn + 1);   // the next turn's half, cleared
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) hn[i] = 0u;
+        if (nh == 0.0 || nJ == 0) {   // wakefield.jl:305-311
+            for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+                put_potential(B, i, 0.0);
+                sP[i] = 0.0;
+            }
+        } else {
+            const double inv_nh = 1.0 / nh;
+            auto dens = [&](int i) {   // line_density from the staged counts
+                if (fl & F_LAG) i = nb - 1 - i;
+                return sLam[i] * ((double)sHist[i] * inv_nh);
+            };
+            for (int u = threadIdx.x; u < nA * F; u += blockDim.x) {   // load_packed, every block
+                const int a = u >> logF, m = u & (F - 1);
+                double re = 0.0, im = 0.0;
+                if (m < Bk) {
+                    const int i = a * Bk + m;
+                    if (i < nb) re = dens(i);
+                    if (a >= 1) im = dens(i - Bk);
+                }
+                sX[a * F + bitrev(m, logF)] = make_double2(re, im);
+            }
+            fft_dit_forward(sX, F, logF, tw, nA);
+            double2 *sR = sX;   // the spectra to invert
+            if (nJ <= 1) {      // k_conv_fused's product
+                const double2 *What = reinterpret_cast<const double2 *>(B.What);
+                for (int u = threadIdx.x; u < nA * F; u += blockDim.x)
+                    sX[u] = cmul(sX[u], What[u & (F - 1)]);
+            } else {
+                for (int u = threadIdx.x; u < nA * F; u += blockDim.x) {
+                    const int o = u >> logF, k = u & (F - 1);
+                    const int jmax = min(o, nJ - 1);
+                    double2 acc = make_double2(0.0, 0.0);
+                    for (int j = 0; j <= jmax; ++j) {
+                        const double2 t = cmul(sX[(o - j) * F + k], sW[j * F + k]);
+                        acc = make_double2(acc.x + t.x, acc.y + t.y);
+                    }
+                    sY[u] = acc;
+                }
+                sR = sY;
+            }
+            fft_dif_inverse(sR, F, logF, tw, nA);
+            const double scale = cur / (double)F;
+            for (int u = threadIdx.x; u < nA * Bk; u += blockDim.x) {   // store_block, every block
+                const int o = u / Bk, m = u - o * Bk;
+                const int i = o * Bk + m;
+                if (i >= nb) continue;
+                const double2 y0 = sR[o * F + bitrev(m, logF)];
+                const double2 y1 = sR[o * F + bitrev(m + Bk, logF)];
+                const int k = (fl & F_LAG) ? nb - 1 - i : i;
+                const double v = (y0.x + y1.y) * scale;
+                put_potential(B, k, v);
+                sP[k] = v;
+            }
+        }
+    }
+    __syncthreads();   // P in LDS before the deposit's sum reads it
+    double t = 0.0;
+    if (cic) {
+        const double d = cic_psum_lds(B, sN, sH, sP, s_red);
+        t = B.nred == 1 ? d : 0.0 + d;
+    }
+    if (threadIdx.x == 0) {
+        finish_on(B, t, sc);
+        *B.sc = sc;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bin_whole(const BunchDev *__restrict__ bds,
+                                                    const int32_t *__restrict__ list) {
+    bin_phase_whole(bds[list[blockIdx.x]]);
 }
 
 __global__ void k_turn_scalars(const BunchDev *__restrict__ bd, double E0, TurnScalars *out) {
@@ -2255,7 +2443,8 @@ hipError_t configure_kernels(int lds) {
                          (const void *)k_conv_fwd, (const void *)k_conv_inv,
                          (const void *)k_conv_spec, (const void *)k_conv_out,
                          (const void *)k_fft_wake, (const void *)k_stage_hist,
-                         (const void *)k_interp_sum, (const void *)k_interp_sum_deep};
+                         (const void *)k_interp_sum, (const void *)k_interp_sum_deep,
+                         (const void *)k_bin_whole};
     hipError_t last = hipSuccess;
     for (const void *f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -2351,6 +2540,13 @@ hipError_t launch_bin_reduce(const BunchDev *bds, const int32_t *items, int nite
 hipError_t launch_bin_reduce_whole(const BunchDev *bds, const int32_t *list, int nb, hipStream_t s) {
     if (nb <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_bin_reduce_whole, dim3(nb), dim3(1024), 0, s, bds, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_whole(const BunchDev *bds, const int32_t *list, int nb, size_t lds,
+                            hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bin_whole, dim3(nb), dim3(1024), lds, s, bds, list);
     return hipGetLastError();
 }
 

@@ -96,6 +96,33 @@ hipError_t launch_bin_reduce_whole(const BunchDev *bds, const int32_t *bunch_lis
                                    hipStream_t s);
 hipError_t launch_finalize_whole(const BunchDev *bds, const int32_t *bunch_list, int nb,
                                  hipStream_t s);
+// The whole bin phase (the reduction, the convolution and the finalize) of
+// each listed bunch in one 1024-thread workgroup: the same bits as the
+// launches of those phases.  For bunches of at most kBinWholeMaxTiles tiles
+// without interpolation sums; lds = the largest bin_whole_layout(...).bytes
+// among them.
+constexpr int kBinWholeMaxTiles = 256;
+// LDS of k_bin_whole for one bunch (byte offsets, 16-aligned): the
+// convolution's X[nA][F] (and for nJ > 1 FFT(W)[nJ][F] and Y[nA][F]), then
+// for a wake bunch this turn's counts[nb], the line-density kernel[nb] and
+// the potential[nb], then for a deposit bunch its hi/lo[2 nb] and n[nb + 1].
+struct BinWholeLayout {
+    size_t hist, lam, pot, cic, bytes;
+};
+__host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+__host__ __device__ inline BinWholeLayout bin_whole_layout(int nb, int nA, int nJ, int F,
+                                                           bool cic) {
+    BinWholeLayout L;
+    const size_t conv = nb > 0 ? (size_t)(nJ <= 1 ? nA : 2 * nA + nJ) * (size_t)F * 16 : 0;
+    L.hist = conv;
+    L.lam = L.hist + align16((size_t)nb * 4);
+    L.pot = L.lam + (size_t)nb * 8;
+    L.cic = L.pot + (size_t)nb * 8;
+    L.bytes = L.cic + (cic ? align16((size_t)nb * 16 + ((size_t)nb + 1) * 4) : 0);
+    return L;
+}
+hipError_t launch_bin_whole(const BunchDev *bds, const int32_t *bunch_list, int nb, size_t lds,
+                            hipStream_t s);
 // Broadband resonators of a lag-indexed wake table, (R_s [Ohm], f_r [Hz], Q)
 // each, passed by value.
 constexpr int kMaxResonators = 8;
