@@ -48,7 +48,7 @@ class HmOptions(C.Structure):
         ("charge", C.c_double), ("n_resonators", C.c_int32), ("timing", C.c_int32),
         ("resonators", _dp), ("fft_block", C.c_int32), ("block_size", C.c_int32),
         ("monitor", _dp), ("serial", C.c_int32), ("fom_stats", C.c_int32),
-        ("tile_pairs", C.c_int32), ("run_pairs", C.c_int32),
+        ("tile_pairs", C.c_int32), ("run_pairs", C.c_int32), ("bin_whole", C.c_int32),
     ]
 
 

@@ -165,6 +165,7 @@ class TrackingOptions:
     fom_stats: bool = False              # final two-pass means and cor(z, ΔE) (figures of merit)
     tile_pairs: int = 0                  # pairs per thread per tile of the turn pass (0: tuned/rule)
     run_pairs: int = 0                   # cost of a run of tiles when balancing the pass (0: tuned/rule)
+    bin_whole: int = 0                   # bin phase of a small bunch in one workgroup: 0 rule, -1 never
 
     def _c(self, keep: list, n: int, n_turns: int, nbins: int) -> HmOptions:
         o = HmOptions()
@@ -205,6 +206,7 @@ class TrackingOptions:
         # else the library's rules
         o.tile_pairs = int(self.tile_pairs) or _tune.tile_pairs_for(n)
         o.run_pairs = int(self.run_pairs) or _tune.run_pairs_for(nbins)
+        o.bin_whole = int(self.bin_whole)
         return o
 
 

@@ -266,6 +266,8 @@ hm_status validate(const hm_params *p, const hm_options *o, int64_t n, int64_t n
         return fail(HM_ERR_ARG, "block_size must be 0 or %d", kPassThreads);
     if (o && (o->tile_pairs < 0 || o->tile_pairs > (1 << 20) || o->run_pairs < 0))
         return fail(HM_ERR_ARG, "tile_pairs must be in [0, 2^20] and run_pairs >= 0");
+    if (o && (o->bin_whole < -1 || o->bin_whole > 0))
+        return fail(HM_ERR_ARG, "bin_whole must be 0 or -1");
     return HM_OK;
 }
 
@@ -414,7 +416,7 @@ hm_status Batch::build_tables() {
             const bool dep = wake && (f & F_E0) && (f & F_CIC);
             const size_t lds =
                 bin_whole_layout(wake ? q.d.nbins : 0, q.d.nA, q.d.nJ, q.d.F, dep).bytes;
-            if (q.d.ntiles <= kBinWholeMaxTiles && lds <= kBinWholeMaxLds &&
+            if (q.o.bin_whole == 0 && q.d.ntiles <= kBinWholeMaxTiles && lds <= kBinWholeMaxLds &&
                 (!wake || !(f & F_E0) || (f & F_CIC))) {
                 bw.push_back((int32_t)i);
                 bw_lds_ = std::max(bw_lds_, lds);

@@ -27,10 +27,11 @@ struct HmOptions
     delta_sigma::Float64; bin_half_width_sigmas::Float64; charge::Float64
     n_resonators::Int32; timing::Int32; resonators::Ptr{Float64}; fft_block::Int32; block_size::Int32
     monitor::Ptr{Float64}; serial::Int32; fom_stats::Int32; tile_pairs::Int32; run_pairs::Int32
+    bin_whole::Int32
 end
 HmOptions(; seed::Integer=0, serial::Bool=false) =
     HmOptions(C_NULL, UInt64(seed), 0, C_NULL, 0.0, 0.0, 0.0, 0, 0, C_NULL, 0, 0, C_NULL, Int32(serial),
-              0, 0, 0)
+              0, 0, 0, 0)
 
 struct HmResult
     sigma_E::Float64; sigma_z::Float64; E0::Float64; turns_done::Int64

@@ -94,6 +94,10 @@ typedef struct {
                                    (staging and flushing its LDS) when the turn pass's
                                    workgroup ranges are balanced; 0 = 256 + nbins/4.  Changes
                                    no result */
+    int32_t bin_whole;          /* the bin phase of a small bunch (reduction, convolution, E0
+                                   update) in one workgroup: 0 = wherever it fits (at most 256
+                                   tiles, no interpolation sums, its tables in 144 KB of LDS),
+                                   -1 = never (a launch per phase).  Changes no result */
 } hm_options;
 
 /* Return of longitudinal_evolve! (evolution.jl:470) plus diagnostics. */

@@ -23,7 +23,7 @@ int main(void) {
     F(hm_options, charge); F(hm_options, n_resonators); F(hm_options, timing);
     F(hm_options, resonators); F(hm_options, fft_block); F(hm_options, block_size);
     F(hm_options, monitor); F(hm_options, serial); F(hm_options, fom_stats);
-    F(hm_options, tile_pairs); F(hm_options, run_pairs);
+    F(hm_options, tile_pairs); F(hm_options, run_pairs); F(hm_options, bin_whole);
     Z(hm_result);
     F(hm_result, sigma_E); F(hm_result, sigma_z); F(hm_result, E0); F(hm_result, turns_done);
     F(hm_result, drift_skipped); F(hm_result, ms_total); F(hm_result, ms_pass_avg);

@@ -265,3 +265,32 @@ def test_large_bunch_matches_oracle_for_three_turns(dev):
     print(f"1e7: normwise z={ez:.2e} dE={ee:.2e}", flush=True)
     assert ez <= COORD_RTOL and ee <= COORD_RTOL
     assert E0 == pytest.approx(rr.E0, rel=1e-14)
+
+
+@pytest.mark.parametrize("n,nbins,flags,lag", [
+    (30_001, 1024, (True, True, True, True, True), True),    # spectral sums, deposit, resonators
+    (4_097, 256, (True, True, True, True, True), False),     # one block (the fused product)
+    (10_000, 512, (True, False, False, True, True), True),   # wake without the E0 update
+    (5_000, 64, (False, True, True, True, True), False),     # E0 update without the wake
+    (2, 2, (True, True, True, True, True), True),            # the smallest grid
+])
+def test_bin_phase_in_one_workgroup_equals_the_launches(dev, n, nbins, flags, lag):
+    """A small bunch's bin phase runs in one workgroup (k_bin_whole) unless
+    bin_whole = -1 asks for a launch per phase: the same expressions of the
+    same operands, so the same bits, monitor included."""
+    p = H.SimulationParameters.from_dict(W.ring_3gev_lattice(n_turns=40, flags=flags))
+    z0, e0 = O.generate_gaussian(31, 0.0, 4e-3, 0.0, 3e6, n)
+    outs = []
+    for bw in (0, -1):
+        b = H.DeviceBunch(n, dev)
+        b.upload(z0, e0)
+        kw = dict(resonators=[W.RESONATOR_BB], charge=2e-9) if lag else {}
+        r = b.evolve(p, nbins, H.TrackingOptions(seed=3, bin_half_width_sigmas=8.0, monitor=True,
+                                                 bin_whole=bw, **kw))
+        outs.append((*b.download(), r))
+        b.close()
+    (za, ea, ra), (zb, eb, rb) = outs
+    np.testing.assert_array_equal(za, zb)
+    np.testing.assert_array_equal(ea, eb)
+    assert (ra.sigma_E, ra.sigma_z, ra.E0, ra.turns_done) == (rb.sigma_E, rb.sigma_z, rb.E0, rb.turns_done)
+    np.testing.assert_array_equal(ra.monitor, rb.monitor)
