@@ -21,8 +21,10 @@ of a CU, so it is one 1024-thread workgroup per CU.
 the table beside this file; ``TrackingOptions`` fields left at 0 take their
 values from it, and keys it does not list keep the library's own rules.
 Each value depends only on the bin count or the bunch size, so a bunch gets
-the same tiling, and the same bits, alone, in a batch and in a served mix
-(the run cost and the FFT block change no result at all).
+the same tiling, and the same bits, alone, in a batch and in a served mix.
+The run cost changes no result; the FFT block changes the potential by
+rounding only (a different blocking of the same convolution), the tiling
+the order of the per-tile sums.
 """
 from __future__ import annotations
 
