@@ -182,6 +182,23 @@ def tune_runs(candidates=(64, 256, 512, 1024, 1280, 2048, 4096, 8192), turns: in
     return {nbins: min(times, key=times.get)}
 
 
+def merge_tables(table: dict, fft=None, tiles=None, tile_sizes=(), runs=None) -> dict:
+    """The table after sweeps: a sweep replaces the entries it measured and
+    keeps the others; the tile sweep also drops the size classes it measured
+    where the rule won (its result lists only the classes a candidate won).
+    Keys as strings, in ascending order, ready for JSON."""
+    out = {name: dict(t) for name, t in table.items()}
+    if fft is not None:
+        out["fft_block"].update(fft)
+    if tiles is not None:
+        classes = {size_class(int(n)) for n in tile_sizes}
+        out["tile_pairs"] = {k: v for k, v in out["tile_pairs"].items() if k not in classes}
+        out["tile_pairs"].update(tiles)
+    if runs is not None:
+        out["run_pairs"].update(runs)
+    return {name: {str(k): v for k, v in sorted(t.items())} for name, t in out.items()}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--what", nargs="*", default=["fft", "tiles", "runs"],
@@ -193,13 +210,12 @@ def main() -> int:
     ap.add_argument("--write", action="store_true", help="write the table")
     ap.add_argument("--out", default=TABLE, help="where --write puts the table")
     args = ap.parse_args()
-    table = {name: {str(k): v for k, v in sorted(t.items())} for name, t in _tables().items()}
-    if "fft" in args.what:
-        table["fft_block"] = {str(k): v for k, v in sorted(tune_fft(args.nbins).items())}
-    if "tiles" in args.what:
-        table["tile_pairs"] = {str(k): v for k, v in sorted(tune_tiles(args.sizes).items())}
-    if "runs" in args.what:
-        table["run_pairs"] = {str(k): v for k, v in sorted(tune_runs().items())}
+    table = merge_tables(
+        _tables(),
+        fft=tune_fft(args.nbins) if "fft" in args.what else None,
+        tiles=tune_tiles(args.sizes) if "tiles" in args.what else None,
+        tile_sizes=args.sizes,
+        runs=tune_runs() if "runs" in args.what else None)
     print(json.dumps(table))
     if args.write:
         import haissinskijulia_amd as H

@@ -69,3 +69,15 @@ def test_committed_table_is_well_formed():
             assert int(k) >= 0 and int(v) > 0
     for k, v in table.get("fft_block", {}).items():
         assert int(v) & (int(v) - 1) == 0                # powers of two
+
+
+def test_a_sweep_merges_into_the_table():
+    base = {"fft_block": {256: 256, 1024: 64, 65536: 512}, "tile_pairs": {19: 4, 22: 4, 26: 8},
+            "run_pairs": {4096: 8192}}
+    # an FFT sweep of two bin counts keeps the third; a tile sweep of 1e6 and
+    # 1e8 (classes 19 and 26) drops 26 where the rule won and keeps 22
+    out = tune.merge_tables(base, fft={256: 64, 2048: 128}, tiles={19: 2}, tile_sizes=[1e6, 1e8])
+    assert out["fft_block"] == {"256": 64, "1024": 64, "2048": 128, "65536": 512}
+    assert out["tile_pairs"] == {"19": 2, "22": 4}
+    assert out["run_pairs"] == {"4096": 8192}
+    assert base["fft_block"][256] == 256   # the input is left as it was
