@@ -2035,9 +2035,9 @@ __device__ inline double cic_psum_lds(const BunchDev &B, const uint32_t *sN,
 // The bin phase of a small bunch in one 1024-thread workgroup, in place of
 // k_bin_reduce, the convolution kernels and k_finalize.  Everything it reads
 // is staged first, side by side (the turn scalars in thread 0's registers,
-// this turn's counts, the line-density kernel, the wake spectra, the
-// deposit), so the phase waits on memory about twice instead of once per
-// step.  Then: the pass sums reduced as whole_sum does (the sliced form's
+// this turn's counts, the line-density kernel, the wake spectra, the FFT
+// twiddles, the deposit), so the phase waits on memory about twice instead
+// of once per step.  Then: the pass sums reduced as whole_sum does (the sliced form's
 // bits) into the turn scalars; the line density of every block packed and
 // transformed side by side (nA blocks of F points); the spectral sums Y_o =
 // sum_j X_{o-j} FFT(W_j) added in ascending j from zero, as k_conv_inv adds
@@ -2059,6 +2059,7 @@ __device__ inline void bin_phase_whole(const BunchDev &B) {
     uint32_t *sHist = reinterpret_cast<uint32_t *>(s_bw + L.hist);
     double *sLam = reinterpret_cast<double *>(s_bw + L.lam);
     double *sP = reinterpret_cast<double *>(s_bw + L.pot);
+    double2 *sTw = reinterpret_cast<double2 *>(s_bw + L.tw);
     unsigned long long *sH = reinterpret_cast<unsigned long long *>(s_bw + L.cic);
     uint32_t *sN = reinterpret_cast<uint32_t *>(sH + 2 * nb);
     TurnScalars sc;
@@ -2074,6 +2075,8 @@ __device__ inline void bin_phase_whole(const BunchDev &B) {
             const double2 *What = reinterpret_cast<const double2 *>(B.What);
             for (int u = threadIdx.x; u < nJ * F; u += blockDim.x) sW[u] = What[u];
         }
+        const double2 *twg = reinterpret_cast<const double2 *>(B.tw);
+        for (int u = threadIdx.x; u < F / 2; u += blockDim.x) sTw[u] = twg[u];
         if (cic) {
             const unsigned long long *C = B.cic;
             const uint32_t *N = reinterpret_cast<const uint32_t *>(B.cic + 2 * nb);
@@ -2093,7 +2096,7 @@ __device__ inline void bin_phase_whole(const BunchDev &B) {
     __syncthreads();
     if (wake) {
         const double nh = s_nh, cur = s_cur;
-        const double2 *tw = reinterpret_cast<const double2 *>(B.tw);
+        const double2 *tw = sTw;   // the twiddles from LDS: no global load per FFT stage
         uint32_t *hn = hist_of(B, turn + 1);   // the next turn's half, cleared
         for (int i = threadIdx.x; i < nb; i += blockDim.x) hn[i] = 0u;
         if (nh == 0.0 || nJ == 0) {   // wakefield.jl:305-311

@@ -104,10 +104,11 @@ hipError_t launch_finalize_whole(const BunchDev *bds, const int32_t *bunch_list,
 constexpr int kBinWholeMaxTiles = 256;
 // LDS of k_bin_whole for one bunch (byte offsets, 16-aligned): the
 // convolution's X[nA][F] (and for nJ > 1 FFT(W)[nJ][F] and Y[nA][F]), then
-// for a wake bunch this turn's counts[nb], the line-density kernel[nb] and
-// the potential[nb], then for a deposit bunch its hi/lo[2 nb] and n[nb + 1].
+// for a wake bunch this turn's counts[nb], the line-density kernel[nb], the
+// potential[nb] and the FFT twiddles[F/2], then for a deposit bunch its
+// hi/lo[2 nb] and n[nb + 1].
 struct BinWholeLayout {
-    size_t hist, lam, pot, cic, bytes;
+    size_t hist, lam, pot, tw, cic, bytes;
 };
 __host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
 __host__ __device__ inline BinWholeLayout bin_whole_layout(int nb, int nA, int nJ, int F,
@@ -117,7 +118,8 @@ __host__ __device__ inline BinWholeLayout bin_whole_layout(int nb, int nA, int n
     L.hist = conv;
     L.lam = L.hist + align16((size_t)nb * 4);
     L.pot = L.lam + (size_t)nb * 8;
-    L.cic = L.pot + (size_t)nb * 8;
+    L.tw = L.pot + align16((size_t)nb * 8);
+    L.cic = L.tw + (nb > 0 ? (size_t)(F / 2) * 16 : 0);
     L.bytes = L.cic + (cic ? align16((size_t)nb * 16 + ((size_t)nb + 1) * 4) : 0);
     return L;
 }
