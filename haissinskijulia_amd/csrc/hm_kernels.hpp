@@ -117,8 +117,8 @@ __host__ __device__ inline BinWholeLayout bin_whole_layout(int nb, int nA, int n
     const size_t conv = nb > 0 ? (size_t)(nJ <= 1 ? nA : 2 * nA + nJ) * (size_t)F * 16 : 0;
     L.hist = conv;
     L.lam = L.hist + align16((size_t)nb * 4);
-    L.pot = L.lam + (size_t)nb * 8;
-    L.tw = L.pot + align16((size_t)nb * 8);
+    L.pot = L.lam + align16((size_t)nb * 8);
+    L.tw = L.pot + align16((size_t)nb * 8);   // double2: every region starts 16-aligned
     L.cic = L.tw + (nb > 0 ? (size_t)(F / 2) * 16 : 0);
     L.bytes = L.cic + (cic ? align16((size_t)nb * 16 + ((size_t)nb + 1) * 4) : 0);
     return L;

@@ -273,6 +273,7 @@ def test_large_bunch_matches_oracle_for_three_turns(dev):
     (10_000, 512, (True, False, False, True, True), True),   # wake without the E0 update
     (5_000, 64, (False, True, True, True, True), False),     # E0 update without the wake
     (2, 2, (True, True, True, True, True), True),            # the smallest grid
+    (1_001, 7, (True, True, True, True, True), True),        # an odd bin count (LDS layout)
 ])
 def test_bin_phase_in_one_workgroup_equals_the_launches(dev, n, nbins, flags, lag):
     """A small bunch's bin phase runs in one workgroup (k_bin_whole) unless
