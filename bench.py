@@ -136,6 +136,70 @@ def cpu_baseline(wl: dict, z0, e0, samples_all: int, turns_all: int, samples_one
     return out
 
 
+def cpu_beside(H, W, dev, scan_gpu) -> list:
+    """The oracle beside the GPU path on the other workloads' shapes, each a
+    one-turn warm-up and one timed run at P = the cores this process may use
+    (up to 16): ring3_1e5 in full (1000 turns), ring3_1e7 truncated to 2
+    turns (per-turn rates), and the 256-bunch scan truncated to 10 turns and
+    run as 256 plain runs in sequence.  Each row also gives the GPU's rate on
+    the same shape from this run (the two single bunches timed here, the scan
+    from its own leg).  The timed code is the oracle, the C restatement of
+    the reference loop, not the Julia reference."""
+    import numpy as np
+    from oracle import oracle as O
+    _, _, avail = host_cpu()
+    cores = max(1, min(16, avail))
+    rows = []
+    for name, t_cpu, t_gpu in (("ring3_1e5", 1000, 1000), ("ring3_1e7", 2, 200)):
+        wl = W.WORKLOADS[name]
+        n, nbins = wl["n"], wl["nbins"]
+        z0, e0 = initial_bunch(H, W, dev, wl)
+        kw = dict(resonators=wl["resonators"], charge=wl["charge"],
+                  bin_half_width_sigmas=wl["half_width"])
+        O.evolve(z0, e0, nbins, W.ring_3gev_lattice(n_turns=1), seed=wl["seed"], nranks=cores, **kw)
+        t0 = time.perf_counter()
+        _, _, r = O.evolve(z0, e0, nbins, W.ring_3gev_lattice(n_turns=t_cpu), seed=wl["seed"],
+                           nranks=cores, **kw)
+        dt_cpu = time.perf_counter() - t0
+        assert r.status == 0
+        b = H.DeviceBunch(n, dev)
+        b.upload(z0, e0)
+        b.evolve(H.SimulationParameters.from_dict(W.ring_3gev_lattice(n_turns=20)), nbins,
+                 tracking_options(wl, wl["seed"]))
+        b.upload(z0, e0)
+        t0 = time.perf_counter()
+        b.evolve(H.SimulationParameters.from_dict(W.ring_3gev_lattice(n_turns=t_gpu)), nbins,
+                 tracking_options(wl, wl["seed"]))
+        dt_gpu = time.perf_counter() - t0
+        b.close()
+        rows.append({"workload": f"{name}: {n:.0e} particles, {nbins} bins",
+                     "cpu_particle_turns_per_s": n * t_cpu / dt_cpu, "cpu_turns": t_cpu,
+                     "gpu_particle_turns_per_s": n * t_gpu / dt_gpu, "gpu_turns": t_gpu,
+                     "cores": cores})
+        log(f"cpu beside: {name}: oracle {n * t_cpu / dt_cpu:.3e}, GPU {n * t_gpu / dt_gpu:.3e} "
+            f"particle-turns/s")
+    specs = W.scan_bunches()
+    t_cpu, total, dt_cpu = 10, 0, 0.0
+    for sp in specs:   # a Gaussian bunch of the bunch's size (the timing needs no more)
+        rng = np.random.default_rng(sp["seed"])
+        z0 = rng.normal(0.0, W.SIGMA_Z, sp["n"])
+        e0 = rng.normal(0.0, W.SIGMA_E, sp["n"])
+        kw = dict(resonators=sp["resonators"], charge=sp["charge"],
+                  bin_half_width_sigmas=sp["half_width"])
+        t0 = time.perf_counter()
+        _, _, r = O.evolve(z0, e0, sp["nbins"], W.ring_3gev_lattice(t_cpu, voltage=sp["V"]),
+                           seed=sp["seed"] + 1, nranks=cores, **kw)
+        dt_cpu += time.perf_counter() - t0
+        total += sp["n"] * t_cpu
+    rows.append({"workload": f"the 256-bunch scan ({sum(sp['n'] for sp in specs):.3e} particles, "
+                             f"4096 bins), 256 plain runs in sequence",
+                 "cpu_particle_turns_per_s": total / dt_cpu, "cpu_turns": t_cpu,
+                 "gpu_particle_turns_per_s": scan_gpu, "gpu_turns": None, "cores": cores})
+    log(f"cpu beside: scan: oracle {total / dt_cpu:.3e} particle-turns/s (GPU batched "
+        f"{scan_gpu if scan_gpu is None else f'{scan_gpu:.3e}'})")
+    return rows
+
+
 def _pct(x, q):
     import numpy as np
     return float(np.percentile(np.asarray(x, dtype=float), q)) if len(x) else None
@@ -346,6 +410,8 @@ def main() -> int:
                     help="fresh-copy runs at P = 1 (median); 0 skips them")
     ap.add_argument("--cpu-turns-p1", type=int, default=1, help="turns of each P = 1 run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-beside", action="store_true",
+                    help="skip the oracle on the other workloads' shapes (about 20 s)")
     ap.add_argument("--scan-turns", type=int, default=5000,
                     help="turns of the 256-bunch scan (batched vs plain, about 80 s at 5000); "
                          "0 skips it")
@@ -405,6 +471,10 @@ def main() -> int:
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} particle-turns/s")
     else:
         out["cpu_baseline"] = None
+    if not args.no_cpu_baseline and not args.no_cpu_beside:
+        log("cpu beside the GPU on the other workloads (oracle) ...")
+        out["cpu_beside"] = cpu_beside(H, W, dev, (out.get("scan") or {}).get(
+            "batched_particle_turns_per_s"))
     print(json.dumps(out), flush=True)
     dev.close()
     return 0
