@@ -330,23 +330,59 @@ static void deal_tiles(const std::vector<int32_t> &tiles, const int32_t *ranges,
     }
 }
 
-// Splits tiles with weights w (pairs) into `grid` contiguous ranges of
-// about equal weight; appends grid + 1 boundaries (none for grid = 0).
-static void balanced_ranges(const std::vector<int64_t> &w, int grid, std::vector<int32_t> &out) {
+// Contiguous ranges of the tile table for `grid` workgroups, [out[g],
+// out[g+1]), that make the largest range's load as small as it can be: a
+// range's load is its tiles' weights plus, where run costs are given, the
+// cost of every run it starts (its first tile, and each tile whose bunch
+// differs from the previous tile's: the workgroup stages and flushes that
+// bunch's LDS once per run).  The smallest feasible bound is found by
+// bisection, each bound checked by filling ranges greedily in order; the fill
+// at that bound gives the ranges, and workgroups beyond the last range get
+// empty ones.  A workgroup's time is about its load, and the launch lasts as
+// long as the largest.
+static void balanced_ranges(const std::vector<int64_t> &w, const std::vector<int64_t> *run_cost,
+                            const std::vector<int32_t> &bunch, int grid, std::vector<int32_t> &out) {
     if (grid <= 0) return;
-    int64_t total = 0;
-    for (int64_t x : w) total += x;
     const int nt = (int)w.size();
-    out.push_back(0);
-    int t = 0;
-    int64_t acc = 0;
-    for (int g = 1; g < grid; ++g) {
-        // the first tile at which the running weight reaches g/grid of the total
-        const int64_t goal = (int64_t)(((__int128)total * g) / grid);
-        while (t < nt && acc + w[(size_t)t] <= goal) acc += w[(size_t)t++];
-        out.push_back((int32_t)t);
+    auto cost = [&](int t, bool starts) {
+        return w[(size_t)t] + (run_cost && starts ? (*run_cost)[(size_t)t] : 0);
+    };
+    // the ranges the greedy fill needs under bound M (grid + 1: infeasible);
+    // with b, their starts after the first
+    auto fill = [&](int64_t M, std::vector<int32_t> *b) {
+        int count = 0;
+        int64_t load = 0;
+        bool open = false;
+        for (int t = 0; t < nt; ++t) {
+            int64_t c = cost(t, !open || bunch[(size_t)t] != bunch[(size_t)t - 1]);
+            if (open && load + c > M) {
+                if (b) b->push_back((int32_t)t);
+                ++count;
+                load = 0;
+                c = cost(t, true);
+            }
+            open = true;
+            if (c > M || count >= grid) return grid + 1;
+            load += c;
+        }
+        return count + (open ? 1 : 0);
+    };
+    int64_t lo = 0, hi = 0;
+    for (int t = 0; t < nt; ++t) {
+        lo = std::max(lo, cost(t, true));
+        hi += cost(t, true);
     }
-    out.push_back((int32_t)nt);
+    while (lo < hi) {   // the smallest M with fill(M) <= grid (hi always is)
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (fill(mid, nullptr) <= grid) hi = mid;
+        else lo = mid + 1;
+    }
+    const size_t base = out.size();
+    out.push_back(0);
+    std::vector<int32_t> b;
+    fill(lo, &b);
+    for (int32_t x : b) out.push_back(x);
+    while (out.size() < base + (size_t)grid + 1) out.push_back((int32_t)nt);
 }
 
 Batch::~Batch() {
@@ -469,18 +505,19 @@ hm_status Batch::build_tables() {
     // Workgroup ranges: contiguous runs of the tile table with equal numbers
     // of pairs (tiles of different bunches differ in size), for the pass and
     // for the interpolation sums (which only count wake + update_E0 bunches).
-    std::vector<int64_t> wpass((size_t)ntiles_), winterp((size_t)ntiles_);
+    std::vector<int64_t> wpass((size_t)ntiles_), winterp((size_t)ntiles_), rcost((size_t)ntiles_);
     for (int t = 0; t < ntiles_; ++t) {
         const BunchDev &d = rb_[(size_t)tiles[(size_t)t]].d;
         const int64_t lt = t - d.tile_begin;
         const int64_t pairs = std::min(d.pairs_per_tile, d.npairs - lt * d.pairs_per_tile);
         // a run of a bunch's tiles first stages the bunch's LDS (histogram,
         // deposit, window of P) and ends by flushing it: about as long as
-        // streaming 256 + nbins/4 pairs, charged to the bunch's first tile
+        // streaming 4 pairs per bin (the 256-bunch scan at 4096 bins: best at
+        // 12288-16384), charged to each range once per run
         const RunBunch &q = rb_[(size_t)tiles[(size_t)t]];
-        const int64_t cost = q.o.run_pairs > 0 ? q.o.run_pairs : 256 + (int64_t)d.nbins / 4;
-        const int64_t run = (lt == 0 && (d.flags & F_WAKE)) ? cost : 0;
-        wpass[(size_t)t] = pairs + run;
+        const int64_t cost = q.o.run_pairs > 0 ? q.o.run_pairs : 4 * (int64_t)d.nbins;
+        rcost[(size_t)t] = (d.flags & F_WAKE) ? cost : 0;
+        wpass[(size_t)t] = pairs;
         winterp[(size_t)t] = ((d.flags & F_WAKE) && (d.flags & F_E0) && !(d.flags & F_CIC)) ? pairs : 0;
     }
     const int per_cu = interp_lds_ + 1024 <= (size_t)ctx_->lds_max / 2 ? 2 : 1;
@@ -490,8 +527,8 @@ hm_status Batch::build_tables() {
     // ranges: [pass_grid_ + 1] then [interp_grid_ + 1] boundaries; order: the
     // pass's tile order, then the interpolation sums'
     std::vector<int32_t> ranges, order;
-    balanced_ranges(wpass, pass_grid_, ranges);
-    balanced_ranges(winterp, interp_grid_, ranges);
+    balanced_ranges(wpass, &rcost, tiles, pass_grid_, ranges);
+    balanced_ranges(winterp, nullptr, tiles, interp_grid_, ranges);
     deal_tiles(tiles, ranges.data(), pass_grid_, order);
     deal_tiles(tiles, ranges.data() + pass_grid_ + 1, interp_grid_, order);
     // one staging round for every table and the descriptors

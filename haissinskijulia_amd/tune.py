@@ -10,8 +10,9 @@ Three tables, each timed on the device with HIP events:
   workgroup, the number of per-wave partial sums and the reductions over
   them);
 - ``run_pairs``: the cost, in pairs, that the pass's workgroup balancing
-  charges to a run of one bunch's tiles (staging and flushing its LDS), by
-  bin count, from the batched 256-bunch scan's device time.
+  charges to each run of one bunch's tiles in a workgroup (staging and
+  flushing its LDS), by bin count, from the batched 256-bunch scan's device
+  time.
 
 The threads per block of the turn pass are not a knob: its workgroup holds
 the LDS histogram and the window of the potential and takes the whole LDS
@@ -150,10 +151,10 @@ def tune_tiles(sizes, nbins: int = 4096, turns: int = 40, reps: int = 3) -> dict
     return out
 
 
-def tune_runs(candidates=(64, 256, 512, 1024, 1280, 2048, 4096, 8192), turns: int = 40,
+def tune_runs(candidates=(256, 1280, 2048, 4096, 8192, 12288, 16384, 24576), turns: int = 40,
               reps: int = 3) -> dict:
     """Run cost for the scan's 4096-bin bunches: the batched 256-bunch scan's
-    device time per candidate (the library's rule, 256 + nbins/4 = 1280, among
+    device time per candidate (the library's rule, 4 nbins = 16384, among
     them)."""
     import haissinskijulia_amd as H
     from haissinskijulia_amd import workloads as W

@@ -92,7 +92,7 @@ typedef struct {
                                    bunch agree bit for bit under the same value */
     int32_t run_pairs;          /* the cost, in pairs, charged to a run of this bunch's tiles
                                    (staging and flushing its LDS) when the turn pass's
-                                   workgroup ranges are balanced; 0 = 256 + nbins/4.  Changes
+                                   workgroup ranges are balanced; 0 = 4 nbins.  Changes
                                    no result */
     int32_t bin_whole;          /* the bin phase of a small bunch (reduction, convolution, E0
                                    update) in one workgroup: 0 = wherever it fits (at most 256
