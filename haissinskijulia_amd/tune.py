@@ -65,6 +65,13 @@ def size_class(n: int) -> int:
     return int(round(math.log2(pairs))) if pairs > 0 else 0
 
 
+def rule_tile_pairs(n: int, threads: int = 1024) -> int:
+    """The pairs per thread per tile that the library's own rule gives a bunch
+    of n particles: about 1024 tiles, at least one pair per thread."""
+    pairs_per_tile = -(-((int(n) + 1) // 2) // 1024)       # ceil(pairs / 1024)
+    return max(1, -(-pairs_per_tile // threads))            # ceil(... / threads)
+
+
 def tile_pairs_for(n: int) -> int:
     """The tuned pairs per thread per tile for a bunch of n particles: the
     entry of the nearest tuned size class (at most two classes away), 0 if
@@ -119,9 +126,10 @@ def tune_fft(nbins_list, n: int = 200_000, turns: int = 60, reps: int = 3) -> di
 
 def tune_tiles(sizes, nbins: int = 4096, turns: int = 40, reps: int = 3) -> dict:
     """Pairs per thread per tile by bunch size class: the whole evolve call's
-    device time (set-up, turns and epilogue) for each candidate, the library's
-    rule (0) among them; the rule is kept unless a candidate beats it by more
-    than 1%."""
+    device time (set-up, turns and epilogue) of one bunch for each candidate,
+    the library's rule (0) among them and no candidate that tiles as the rule
+    does.  Returns {size class: {candidate: ms}}; merge_tables turns it into
+    entries."""
     import haissinskijulia_amd as H
     from haissinskijulia_amd import workloads as W
     dev = H.MI355X.default()
@@ -129,7 +137,8 @@ def tune_tiles(sizes, nbins: int = 4096, turns: int = 40, reps: int = 3) -> dict
     for n in sizes:
         n = int(n)
         pairs = (n + 1) // 2
-        cands = [0] + [p for p in (1, 2, 4, 8, 16, 32, 64, 128) if p * 1024 <= max(pairs, 1024)]
+        cands = [0] + [p for p in (1, 2, 4, 8, 16, 32, 64, 128)
+                       if p * 1024 <= max(pairs, 1024) and p != rule_tile_pairs(n)]
         b = H.DeviceBunch(n, dev)
         times = {}
         for tp in cands:
@@ -145,9 +154,7 @@ def tune_tiles(sizes, nbins: int = 4096, turns: int = 40, reps: int = 3) -> dict
             print(f"n {n:10d} tile_pairs {tp or 'rule':>4}: {times[tp] / turns * 1e3:9.2f} us/turn",
                   flush=True)
         b.close()
-        best = min(times, key=times.get)
-        if best != 0 and times[best] < 0.99 * times[0]:
-            out[size_class(n)] = best
+        out[size_class(n)] = times
     return out
 
 
@@ -183,18 +190,27 @@ def tune_runs(candidates=(256, 1280, 2048, 4096, 8192, 12288, 16384, 24576), tur
     return {nbins: min(times, key=times.get)}
 
 
-def merge_tables(table: dict, fft=None, tiles=None, tile_sizes=(), runs=None) -> dict:
+def merge_tables(table: dict, fft=None, tiles=None, runs=None) -> dict:
     """The table after sweeps: a sweep replaces the entries it measured and
-    keeps the others; the tile sweep also drops the size classes it measured
-    where the rule won (its result lists only the classes a candidate won).
-    Keys as strings, in ascending order, ready for JSON."""
+    keeps the others.  For each size class the tile sweep timed (`tiles`, as
+    tune_tiles returns it), a candidate that beats the rule by more than 1%
+    becomes the entry; otherwise the class keeps its entry if the sweep timed
+    that entry within 1% of the rule, and drops it if not.  A neutral entry
+    stays because the sweep times one bunch alone, and the tiling also sets
+    how a batch of many bunches spreads over the workgroups (on the 256-bunch
+    scan, 4 pairs per thread for the bunches near 1e6 particles, neutral
+    alone, takes 14% off the batch's time).  Keys as strings, in ascending
+    order, ready for JSON."""
     out = {name: dict(t) for name, t in table.items()}
     if fft is not None:
         out["fft_block"].update(fft)
-    if tiles is not None:
-        classes = {size_class(int(n)) for n in tile_sizes}
-        out["tile_pairs"] = {k: v for k, v in out["tile_pairs"].items() if k not in classes}
-        out["tile_pairs"].update(tiles)
+    for c, times in (tiles or {}).items():
+        best = min(times, key=times.get)
+        old = out["tile_pairs"].pop(c, None)
+        if best != 0 and times[best] < 0.99 * times[0]:
+            out["tile_pairs"][c] = best
+        elif old is not None and old in times and times[old] <= 1.01 * times[0]:
+            out["tile_pairs"][c] = old
     if runs is not None:
         out["run_pairs"].update(runs)
     return {name: {str(k): v for k, v in sorted(t.items())} for name, t in out.items()}
@@ -215,7 +231,6 @@ def main() -> int:
         _tables(),
         fft=tune_fft(args.nbins) if "fft" in args.what else None,
         tiles=tune_tiles(args.sizes) if "tiles" in args.what else None,
-        tile_sizes=args.sizes,
         runs=tune_runs() if "runs" in args.what else None)
     print(json.dumps(table))
     if args.write:

@@ -327,6 +327,34 @@ def test_tile_knobs_keep_bits_alone_and_batched():
             assert normwise(zy, zx) <= COORD_RTOL and normwise(ey, ex) <= COORD_RTOL
 
 
+def test_the_tuners_rule_is_the_librarys_rule(monkeypatch):
+    """tune.rule_tile_pairs restates the library's default tiling: with no
+    tuned entry, tile_pairs = 0 and tile_pairs = rule_tile_pairs(n) give the
+    same bits in the state, the per-turn monitor and the final moments.  A
+    neighbouring tiling only reorders floating-point sums (the per-tile
+    moments), which in a short run need not change any bit (the histogram
+    and the cloud-in-cell E0 sums are integer), so the test does not ask
+    for a difference there; it prints what differed."""
+    from haissinskijulia_amd import tune
+    monkeypatch.setattr(tune, "_table", {"fft_block": {}, "tile_pairs": {}, "run_pairs": {}})
+    s = dict(n=3_000_001, nbins=2048, V=3.2e6, charge=5e-9, seed=73, flags=(True,) * 5)
+    rule = tune.rule_tile_pairs(s["n"])
+    assert rule == 2        # ceil(ceil(1500001 / 1024) / 1024)
+    out = {}
+    for tp in (0, rule, rule + 1):
+        b = _bunch(s)
+        o = _opts(s)
+        o.tile_pairs, o.monitor = tp, True
+        r = b.evolve(_params(10, s["V"], s["flags"]), s["nbins"], o)
+        out[tp] = [np.array([r.E0, r.sigma_E, r.sigma_z, r.mean_z, r.mean_dE]),
+                   np.asarray(r.monitor), *b.download()]
+        b.close()
+    same = lambda x, y: [bool(np.array_equal(u, v)) for u, v in zip(x, y)]
+    assert same(out[0], out[rule]) == [True] * 4
+    print(f"tile_pairs {rule + 1} against the rule: scalars, monitor, z, dE equal: "
+          f"{same(out[0], out[rule + 1])}", flush=True)
+
+
 def test_closing_a_device_releases_its_bunches_and_servers_first():
     dev = H.MI355X(0)
     b = H.DeviceBunch(1000, dev)

@@ -1,4 +1,4 @@
-"""The convolution-block table (haissinskijulia_amd/tune.py): lookups by bin
+"""The tuned launch tables (haissinskijulia_amd/tune.py): lookups by bin
 count, the library's own rule (0) for counts it does not list, and the
 options' default taking the table's block."""
 import json
@@ -61,6 +61,15 @@ def test_tile_and_run_tables(tmp_path, monkeypatch):
     assert (o.tile_pairs, o.run_pairs) == (8, 100)
 
 
+def test_rule_tile_pairs():
+    # about 1024 tiles of 1024 threads, at least one pair per thread
+    assert tune.rule_tile_pairs(10_000) == 1
+    assert tune.rule_tile_pairs(10_000_000) == 5      # 5e6 pairs: 4883 per tile
+    assert tune.rule_tile_pairs(100_000_000) == 48    # 5e7 pairs: 48829 per tile
+    assert tune.rule_tile_pairs(2 * 1024 * 1024 * 7) == 7
+    assert tune.rule_tile_pairs(2 * 1024 * 1024 * 7 + 2) == 8
+
+
 def test_committed_table_is_well_formed():
     with open(tune.TABLE) as f:
         table = json.load(f)
@@ -72,12 +81,20 @@ def test_committed_table_is_well_formed():
 
 
 def test_a_sweep_merges_into_the_table():
-    base = {"fft_block": {256: 256, 1024: 64, 65536: 512}, "tile_pairs": {19: 4, 22: 4, 26: 8},
-            "run_pairs": {4096: 8192}}
-    # an FFT sweep of two bin counts keeps the third; a tile sweep of 1e6 and
-    # 1e8 (classes 19 and 26) drops 26 where the rule won and keeps 22
-    out = tune.merge_tables(base, fft={256: 64, 2048: 128}, tiles={19: 2}, tile_sizes=[1e6, 1e8])
+    base = {"fft_block": {256: 256, 1024: 64, 65536: 512},
+            "tile_pairs": {17: 8, 19: 4, 22: 4, 26: 8, 28: 2}, "run_pairs": {4096: 8192}}
+    # timings by size class, 0 = the rule: class 19 has a winner (2, 2% under
+    # the rule); 22's entry is neutral (0.5% over the rule) and stays; 26's
+    # entry is 5% slower than the rule and goes; 28's entry was not timed and
+    # goes; 17's the sweep did not measure and it stays; 12 gets no entry
+    tiles = {12: {0: 10.0, 2: 10.05, 4: 10.2},
+             19: {0: 10.0, 2: 9.8, 4: 9.95},
+             22: {0: 10.0, 4: 10.05, 8: 10.5},
+             26: {0: 10.0, 8: 10.5, 64: 9.95},
+             28: {0: 10.0, 64: 10.2}}
+    out = tune.merge_tables(base, fft={256: 64, 2048: 128}, tiles=tiles)
     assert out["fft_block"] == {"256": 64, "1024": 64, "2048": 128, "65536": 512}
-    assert out["tile_pairs"] == {"19": 2, "22": 4}
+    assert out["tile_pairs"] == {"17": 8, "19": 2, "22": 4}
     assert out["run_pairs"] == {"4096": 8192}
     assert base["fft_block"][256] == 256   # the input is left as it was
+    assert base["tile_pairs"][26] == 8
