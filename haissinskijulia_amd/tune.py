@@ -72,43 +72,16 @@ def rule_tile_pairs(n: int, threads: int = 1024) -> int:
     return max(1, -(-pairs_per_tile // threads))            # ceil(... / threads)
 
 
-# The turn pass runs one 1024-thread workgroup per CU: 256 on the MI355X.
-PASS_WORKGROUPS = 256
-# A bunch of at least this many pairs (about 1.7e7 particles) keeps the pass's
-# workgroups busy alone.
-FILLS_DEVICE_PAIRS = 1 << 23
-
-
-def largest_load(n: int, tp: int, workgroups: int = PASS_WORKGROUPS) -> int:
-    """Pairs of the most loaded pass workgroup when a bunch of n particles,
-    cut into tiles of tp pairs per thread, is spread over the workgroups a
-    whole tile at a time."""
-    per_tile = tp * 1024
-    tiles = -(-((int(n) + 1) // 2) // per_tile)
-    return -(-tiles // workgroups) * per_tile
-
-
 def tile_pairs_for(n: int) -> int:
     """The tuned pairs per thread per tile for a bunch of n particles: the
-    entry of the nearest tuned size class (at most two classes away), 0 (the
-    rule) if there is none.  A bunch that fills the device alone takes the
-    entry only if its most loaded workgroup carries at most 2% more pairs
-    than under the rule: an entry tuned at 1e8 particles (64 pairs, 763 tiles
-    for 256 workgroups) would leave a 5e7-particle bunch 382 tiles, a third
-    more on the heaviest workgroup.  Smaller bunches share the device in
-    batches, where the sweep's entries were measured as they are."""
+    entry of the nearest tuned size class (at most two classes away), 0 if
+    there is none."""
     tab = _tables()["tile_pairs"]
     if not tab or n <= 0:
         return 0
     c = size_class(n)
     best = min(tab, key=lambda k: (abs(k - c), k))
-    if abs(best - c) > 2:
-        return 0
-    tp = tab[best]
-    if (n + 1) // 2 >= FILLS_DEVICE_PAIRS and \
-            largest_load(n, tp) > 1.02 * largest_load(n, rule_tile_pairs(n)):
-        return 0
-    return tp
+    return tab[best] if abs(best - c) <= 2 else 0
 
 
 def run_pairs_for(nbins: int) -> int:

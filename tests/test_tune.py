@@ -70,28 +70,6 @@ def test_rule_tile_pairs():
     assert tune.rule_tile_pairs(2 * 1024 * 1024 * 7 + 2) == 8
 
 
-def test_a_large_bunch_takes_an_entry_only_if_it_stays_balanced(tmp_path, monkeypatch):
-    path = tmp_path / "tuning.json"
-    path.write_text(json.dumps({"tile_pairs": {"19": 4, "26": 64}}))
-    monkeypatch.setattr(tune, "TABLE", str(path))
-    monkeypatch.setattr(tune, "_table", None)
-    # 1e8: 763 tiles of 65536 pairs, 3 on the heaviest of 256 workgroups;
-    # the rule's 1018 tiles of 49152, 4 on the heaviest: the same load
-    assert tune.largest_load(100_000_000, 64) == 3 * 65536
-    assert tune.largest_load(100_000_000, 48) == 4 * 49152
-    assert tune.tile_pairs_for(100_000_000) == 64
-    # 5e7 (class 25): 382 tiles, 2 of 65536 on the heaviest against the rule's
-    # 4 of 24576: the rule
-    assert tune.largest_load(50_000_000, 64) == 2 * 65536
-    assert tune.tile_pairs_for(50_000_000) == 0
-    # 1.2e8 (class 26): 916 tiles, 4 of 65536 against 4 of 59392: the rule
-    assert tune.tile_pairs_for(120_000_000) == 0
-    # below 2^23 pairs the entry applies as measured: a 2.5e6 bunch takes 4
-    # pairs (306 tiles), although alone the rule would spread it better
-    assert tune.largest_load(2_500_000, 4) > 1.02 * tune.largest_load(2_500_000, 2)
-    assert tune.tile_pairs_for(2_500_000) == 4
-
-
 def test_committed_table_is_well_formed():
     with open(tune.TABLE) as f:
         table = json.load(f)
