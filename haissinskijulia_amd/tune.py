@@ -128,8 +128,9 @@ def tune_tiles(sizes, nbins: int = 4096, turns: int = 40, reps: int = 3) -> dict
     """Pairs per thread per tile by bunch size class: the whole evolve call's
     device time (set-up, turns and epilogue) of one bunch for each candidate,
     the library's rule (0) among them and no candidate that tiles as the rule
-    does.  Returns {size class: {candidate: ms}}; merge_tables turns it into
-    entries."""
+    does.  The rule runs as its explicit value, since 0 in TrackingOptions
+    would take the table's own entry.  Returns {size class: {candidate: ms}};
+    merge_tables turns it into entries."""
     import haissinskijulia_amd as H
     from haissinskijulia_amd import workloads as W
     dev = H.MI355X.default()
@@ -147,7 +148,8 @@ def tune_tiles(sizes, nbins: int = 4096, turns: int = 40, reps: int = 3) -> dict
                 b.generate(1 + r, W.SIGMA_Z, W.SIGMA_E)
                 res = b.evolve(H.SimulationParameters.from_dict(W.ring_3gev_lattice(turns)), nbins,
                                H.TrackingOptions(seed=2, resonators=[W.RESONATOR_BB], charge=5e-9,
-                                                 bin_half_width_sigmas=8.0, tile_pairs=tp or 0,
+                                                 bin_half_width_sigmas=8.0,
+                                                 tile_pairs=tp or rule_tile_pairs(n),
                                                  fft_block=fft_block_for(nbins)))
                 ms.append(res.ms_total)
             times[tp] = _median(ms)
