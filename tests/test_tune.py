@@ -70,6 +70,40 @@ def test_rule_tile_pairs():
     assert tune.rule_tile_pairs(2 * 1024 * 1024 * 7 + 2) == 8
 
 
+def test_tile_sweep_method(monkeypatch):
+    """The tile sweep on a stand-in device (no GPU): one untimed call first,
+    then the candidates interleaved within each repeat; the rule runs as its
+    explicit value (0 in TrackingOptions would take the table's entry), and
+    no candidate repeats the rule's tiling."""
+    calls = []
+
+    class FakeBunch:
+        def __init__(self, n, dev):
+            self.n = n
+
+        def generate(self, *a):
+            pass
+
+        def evolve(self, params, nbins, opts):
+            calls.append(opts.tile_pairs)
+            return type("R", (), {"ms_total": 10.0 - (opts.tile_pairs == 4)})()
+
+        def close(self):
+            pass
+
+    monkeypatch.setattr(H, "DeviceBunch", FakeBunch)
+    monkeypatch.setattr(H.MI355X, "default", classmethod(lambda cls: None))
+    monkeypatch.setattr(tune, "_table", {"fft_block": {}, "tile_pairs": {22: 64}, "run_pairs": {}})
+    out = tune.tune_tiles([10_000_000], reps=2)
+    rule = tune.rule_tile_pairs(10_000_000)                  # 5
+    cands = [1, 2, 4, 8, 16, 32, 64, 128]
+    assert 0 not in calls and rule not in cands
+    assert calls == [rule] + 2 * ([rule] + cands)            # warm-up, then two interleaved repeats
+    assert out == {22: {0: 10.0, **{c: 10.0 - (c == 4) for c in cands}}}
+    # 4 pairs beat the rule by 10%: the merged table takes it over the old 64
+    assert tune.merge_tables(tune._table, tiles=out)["tile_pairs"] == {"22": 4}
+
+
 def test_committed_table_is_well_formed():
     with open(tune.TABLE) as f:
         table = json.load(f)
